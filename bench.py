@@ -1,0 +1,121 @@
+#!/usr/bin/env python3
+"""Benchmark of the RK4 time step of the electroweak lattice evolution.
+
+One "step" is one full RK4 step (4 stage launches) of the 36-channel
+Higgs-gauge state on an N^3 periodic lattice (default 512^3, FP64), with the
+state resident in HBM before the timed region starts.  The state is a seeded
+synthetic smooth field generated on the device.
+
+Prints ONE JSON line (rank 0):
+  value      = site-updates/s (lattice sites x RK4 steps / second)
+  roofline   = achieved algorithmic HBM bandwidth of the dominant stage kernel
+               against MI355X's 8 TB/s, from HIP-event launch timings
+  cpu_baseline = the CPU oracle (literal C restatement, OpenMP) on a bounded
+               sample of the same workload, on this host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "ewpt-gpu_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline"
+HBM_PEAK = 8.0e12
+# algorithmic HBM bytes per site per stage launch, FP64 (stage-separated RK4
+# with 20 stored derivative channels per stage): reads+writes in channels
+FP64_STAGE_CH = {1: 36 + 20, 2: 36 + 20 + 20, 3: 36 + 16 + 20 + 20, 4: 36 + 60 + 36}
+
+
+def cpu_baseline(n: int = 64, steps: int = 12):
+    """Time the CPU oracle's RK4 (literal C restatement) on n^3 x steps."""
+    from oracle import cref, fields
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    s = fields.smooth_state(n, seed=11)
+    p = fields.params()
+    cref.rk4(s, p, 0.25, 1)  # warm (page-in, thread pool)
+    t0 = time.perf_counter()
+    cref.rk4(s, p, 0.25, steps)
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "site-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle/ewpt_oracle.c FP64 RK4, {n}^3 lattice x {steps} steps, {dt:.1f} s"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512, help="lattice edge")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1:
+        print(f"bench.py: this build runs the hot path on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        return 2
+
+    import ewpt
+    from ewpt import _lib
+
+    prec = _lib.EWPT_FP64 if a.precision == "fp64" else _lib.EWPT_MIXED
+    n = a.n
+    dx = 0.25
+    P = ewpt.Parameters(N=n, dx=dx, gy=0.35).step_params()
+    print(f"bench: {n}^3 {a.precision} variant={a.variant} warmup={a.warmup} steps={a.steps}",
+          file=sys.stderr, flush=True)
+    lat = ewpt.Lattice(n, dx, precision=prec)
+    lat.variant = a.variant
+    lat.init_modes(seed=2024, nmodes=4, kmax=3, amp=0.3)
+    if a.warmup:
+        lat.step(P, a.warmup)
+    stage_ms, total_ms = lat.step_timed(P, a.steps)   # synchronises before returning
+    sites = n ** 3
+    value = sites * a.steps / (total_ms * 1e-3)
+    e = lat.energies(P)
+    lat.close()
+
+    bytes_per = 8 if prec == _lib.EWPT_FP64 else 4
+    stage_bytes = {s: FP64_STAGE_CH[s] * bytes_per * sites for s in (1, 2, 3, 4)}
+    dom = max(range(4), key=lambda s: stage_ms[s]) + 1
+    achieved = stage_bytes[dom] / (stage_ms[dom - 1] * 1e-3)
+    step_bytes = sum(stage_bytes.values())
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": total_ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == _lib.EWPT_FP64 else "f32+f64",
+        "data": "synthetic (seeded smooth Fourier-mode fields generated on the device)",
+        "config": {"workload": f"RK4 step of the 36-channel Higgs-gauge lattice, {n}^3 periodic, "
+                               f"{a.precision}, dx=0.25, dt=dx/10",
+                   "lattice": n, "precision": a.precision, "variant": a.variant},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": f"stage {dom}", "bytes_per_site": stage_bytes[dom] // sites},
+        "step_roofline_frac": value * step_bytes / sites / HBM_PEAK,
+        "stage_ms": stage_ms,
+        "energy_check": float(sum(e[:7])),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

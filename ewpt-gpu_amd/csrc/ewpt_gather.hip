@@ -1,0 +1,353 @@
+// ewpt_gather.hip -- FP64 stage kernel, one site per thread, stencil taps
+// gathered straight from global memory through the caches.
+//
+// This is the simple form of the rk4_step! launch (main_rk4.jl:52-661 with
+// the grid (N/64,N,N) x (64,1,1) of parameters.jl:74,90-92), extended with
+// the RK4 bookkeeping of ewpt_stage.h: the stage input y + c*k is formed on
+// load and stage 4 writes y' directly.  It works for any lattice size >= 4.
+// It also provides the energy and Gauss-law reductions.
+#include "ewpt_internal.h"
+
+namespace ewpt {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+struct Geo {
+    int nx, ny, nz;
+    double inv60dx, invdx2;
+};
+
+// Gather centre values and all derivatives of the stage-S input at (i,j,k).
+template <int S>
+__device__ __forceinline__ void gather_site(const StageBufs<double> &b, const Geo &g, int i, int j, int k,
+                                            double dt, SiteIn<double> &s)
+{
+    const long nx = g.nx, nxy = (long)g.nx * g.ny;
+    const long q = i + nx * j + nxy * k;
+    long ox[7], oy[7], oz[7];
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+        ox[t] = wrap_idx(i + t - 3, g.nx) - i;
+        oy[t] = (wrap_idx(j + t - 3, g.ny) - j) * nx;
+        oz[t] = (wrap_idx(k + t - 3, g.nz) - k) * nxy;
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; c++) s.c[c] = stage_in<S>(b, c, q, dt);
+
+    auto line = [&](int c, const long *o, double v[7]) {
+#pragma unroll
+        for (int t = 0; t < 7; t++) v[t] = (t == 3) ? s.c[c] : stage_in<S>(b, c, q + o[t], dt);
+    };
+    // channels 0..15: gradient and Laplacian
+#pragma unroll 1
+    for (int c = 0; c < 16; c++) {
+        double lap = 0.0, d[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            double v[7];
+            line(c, ax == 0 ? ox : ax == 1 ? oy : oz, v);
+            d[ax] = sten_d1(v[0], v[1], v[2], v[4], v[5], v[6], g.inv60dx);
+            lap += sten_d2(v[0], v[1], v[2], v[3], v[4], v[5], v[6], g.invdx2);
+        }
+        s.lap[c] = lap;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            if (c < 4) s.dphi[c][ax] = d[ax];
+            else if (c < 13) s.dW[(c - 4) / 3][(c - 4) % 3][ax] = d[ax];
+            else s.dY[c - 13][ax] = d[ax];
+        }
+    }
+    // Gamma, Sigma: gradient
+#pragma unroll
+    for (int c = 16; c < 20; c++) {
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            double v[7];
+            line(c, ax == 0 ? ox : ax == 1 ? oy : oz, v);
+            const double d = sten_d1(v[0], v[1], v[2], v[4], v[5], v[6], g.inv60dx);
+            if (c < 19) s.dG[c - 16][ax] = d; else s.dS[ax] = d;
+        }
+    }
+    // divergences of E^a, E^Y: one direction per channel
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        double dv = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            const int c = (a < 3) ? CH_E + 3 * a + ax : CH_EY + ax;
+            double v[7];
+            line(c, ax == 0 ? ox : ax == 1 ? oy : oz, v);
+            dv += sten_d1(v[0], v[1], v[2], v[4], v[5], v[6], g.inv60dx);
+        }
+        if (a < 3) s.divE[a] = dv; else s.divEY = dv;
+    }
+}
+
+template <int S>
+__global__ void __launch_bounds__(BX *BY) stage_gather(StageBufs<double> b, Geo g, Phys P, double dt)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    SiteIn<double> s;
+    gather_site<S>(b, g, i, j, k, dt, s);
+    double o[NK];
+    rhs_point(s, P, o);
+    const long q = i + (long)g.nx * (j + (long)g.ny * k);
+    if constexpr (S < 4) {
+#pragma unroll
+        for (int m = 0; m < NK; m++) b.kout[(long)m * b.n3 + q] = o[m];
+    } else {
+        // y' = k4*(dt/6) + (((k1*(dt/6) + y) + k2*(dt/3)) + k3*(dt/3))
+        const double d6 = dt / 6.0, d3 = dt / 3.0;
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            const double yv = b.y[(long)c * b.n3 + q];
+            double k1, k2, k3, k4;
+            if (c < 16) {
+                const double piv = b.y[(long)(c + 20) * b.n3 + q];
+                k1 = piv;
+                k2 = piv + b.k1[(long)(c + 4) * b.n3 + q] * (dt * 0.5);
+                k3 = piv + b.k2[(long)(c + 4) * b.n3 + q] * (dt * 0.5);
+                k4 = piv + b.k3[(long)(c + 4) * b.n3 + q] * dt;
+            } else {
+                const long m = (long)(c - 16) * b.n3 + q;
+                k1 = b.k1[m]; k2 = b.k2[m]; k3 = b.k3[m]; k4 = o[c - 16];
+            }
+            double n = k1 * d6 + yv;
+            n = k2 * d3 + n;
+            n = k3 * d3 + n;
+            b.ynew[(long)c * b.n3 + q] = k4 * d6 + n;
+        }
+    }
+}
+
+// RHS only: 36 in -> 36 out (the 16 field k's are the momenta copies).
+__global__ void __launch_bounds__(BX *BY) rhs_gather(const double *in, double *out, long n3, Geo g, Phys P)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    StageBufs<double> b{};
+    b.y = in;
+    b.n3 = n3;
+    SiteIn<double> s;
+    gather_site<1>(b, g, i, j, k, 0.0, s);
+    double o[NK];
+    rhs_point(s, P, o);
+    const long q = i + (long)g.nx * (j + (long)g.ny * k);
+#pragma unroll
+    for (int c = 0; c < 16; c++) out[(long)c * n3 + q] = s.c[CH_PI + c];
+#pragma unroll
+    for (int m = 0; m < NK; m++) out[(long)(16 + m) * n3 + q] = o[m];
+}
+
+// ---------------------------------------------------------------- reductions
+
+constexpr int NRED = 16;  // partial slots per block
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Block-reduce nq values (slot q uses min if minmask bit q set), write to part[blk*NRED + q].
+template <int NQ>
+__device__ void block_reduce(double (&v)[NQ], unsigned minmask, double *part)
+{
+    __shared__ double red[(BX * BY) / 64][NQ];
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const double r = (minmask >> q & 1u) ? wave_min(v[q]) : wave_sum(v[q]);
+        if (lane == 0) red[w][q] = r;
+    }
+    __syncthreads();
+    if (tid < NQ) {
+        double r = red[0][tid];
+        for (int ww = 1; ww < (BX * BY) / 64; ww++)
+            r = (minmask >> tid & 1u) ? fmin(r, red[ww][tid]) : r + red[ww][tid];
+        const long blk = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        part[blk * NRED + tid] = r;
+    }
+}
+
+// slots: 0 PE, 1 KE, 2 GE, 3 EEW, 4 MEW, 5 EEY, 6 MEY, 7 sum|phi|, 8 min|phi|,
+// 9 count of non-finite densities.
+// Densities: main_rk4.jl:1341-1342 (PE), 1659 (KE), 1498-1501 (GE),
+// 1817-1820 (EEW), 1978-1981 (MEW), 2139-2140 (EEY), 2298 (MEY).
+__global__ void __launch_bounds__(BX *BY) energy_density(const double *y, long n3, Geo g, Phys P, double *part)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, INFINITY, 0};
+    if (i < g.nx && j < g.ny) {
+        StageBufs<double> b{};
+        b.y = y;
+        b.n3 = n3;
+        SiteIn<double> s;
+        gather_site<1>(b, g, i, j, k, 0.0, s);
+        const double r2 = s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3];
+        const double X = r2 - P.vev * P.vev;
+        v[0] = P.lam * (X * X);
+        v[1] = s.c[CH_PI] * s.c[CH_PI] + s.c[CH_PI + 1] * s.c[CH_PI + 1] + s.c[CH_PI + 2] * s.c[CH_PI + 2] +
+               s.c[CH_PI + 3] * s.c[CH_PI + 3];
+        double D[4][3];
+        cov_derivs(s, P.gw, P.gy, D);
+        double ge = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; a++) ge += D[a][0] * D[a][0] + D[a][1] * D[a][1] + D[a][2] * D[a][2];
+        v[2] = ge;
+        double B[3][3], BYv[3];
+        field_strengths(s, P.gw, B, BYv);
+        double ee = 0.0, me = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) ee += s.c[CH_E + q] * s.c[CH_E + q];
+#pragma unroll
+        for (int a = 0; a < 3; a++) me += B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2];
+        v[3] = 0.5 * ee;
+        v[4] = 0.5 * me;
+        v[5] = 0.5 * (s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2]);
+        v[6] = 0.5 * (BYv[0] * BYv[0] + BYv[1] * BYv[1] + BYv[2] * BYv[2]);
+        const double m = sqrt(r2);
+        v[7] = m;
+        v[8] = m;
+        int bad = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) bad += !isfinite(v[q]);
+        v[9] = (double)bad;
+    }
+    block_reduce<10>(v, 1u << 8, part);
+}
+
+// Gauss residuals (squared), slots 0..7 as documented in ewpt.h.
+__global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3, Geo g, Phys P, double *part)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < g.nx && j < g.ny) {
+        StageBufs<double> b{};
+        b.y = y;
+        b.n3 = n3;
+        SiteIn<double> s;
+        gather_site<1>(b, g, i, j, k, 0.0, s);
+        const double p[4] = {s.c[0], s.c[1], s.c[2], s.c[3]};
+        const double pi[4] = {s.c[CH_PI], s.c[CH_PI + 1], s.c[CH_PI + 2], s.c[CH_PI + 3]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            double na = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < 3; ii++)
+                na -= xprod(a, s.c[CH_W + ii], s.c[CH_W + 3 + ii], s.c[CH_W + 6 + ii], s.c[CH_E + ii],
+                            s.c[CH_E + 3 + ii], s.c[CH_E + 6 + ii]);
+            const double r = s.divE[a] - P.gw * (na + phiT(a, p, pi));
+            v[a] = r * r;
+            const double dW = s.dW[a][0][0] + s.dW[a][1][1] + s.dW[a][2][2];
+            const double cc = s.c[CH_GAM + a] - dW;
+            v[4 + a] = cc * cc;
+        }
+        const double rY = s.divEY - P.gy * phiTY(p, pi);
+        v[3] = rY * rY;
+        const double cY = s.c[CH_SIG] - (s.dY[0][0] + s.dY[1][1] + s.dY[2][2]);
+        v[7] = cY * cY;
+    }
+    block_reduce<8>(v, 0u, part);
+}
+
+// Sum (or min) nblk partials of slot q in a fixed order; one block per slot.
+__global__ void __launch_bounds__(256) finish_reduce(const double *part, long nblk, unsigned minmask, double *res)
+{
+    __shared__ double sh[256];
+    const int q = blockIdx.x;
+    const bool mn = (minmask >> q) & 1u;
+    double acc = mn ? INFINITY : 0.0;
+    for (long b = threadIdx.x; b < nblk; b += 256) {
+        const double v = part[b * NRED + q];
+        acc = mn ? fmin(acc, v) : acc + v;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = mn ? fmin(sh[threadIdx.x], sh[threadIdx.x + st]) : sh[threadIdx.x] + sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) res[q] = sh[0];
+}
+
+Geo make_geo(const ewpt_lattice &L)
+{
+    Geo g;
+    g.nx = L.nx; g.ny = L.ny; g.nz = L.nz;
+    g.inv60dx = 1.0 / (60.0 * L.dx);
+    g.invdx2 = 1.0 / (L.dx * L.dx);
+    return g;
+}
+
+dim3 grid_of(const ewpt_lattice &L) { return dim3((L.nx + BX - 1) / BX, (L.ny + BY - 1) / BY, L.nz); }
+
+}  // namespace
+
+hipError_t launch_stage_gather(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P,
+                               double dt, hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    const dim3 grid = grid_of(L), block(BX, BY, 1);
+    switch (stage) {
+    case 1: stage_gather<1><<<grid, block, 0, st>>>(b, g, P, dt); break;
+    case 2: stage_gather<2><<<grid, block, 0, st>>>(b, g, P, dt); break;
+    case 3: stage_gather<3><<<grid, block, 0, st>>>(b, g, P, dt); break;
+    default: stage_gather<4><<<grid, block, 0, st>>>(b, g, P, dt); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs_gather(const double *in, double *out, const ewpt_lattice &L, const Phys &P, hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    rhs_gather<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(in, out, (long)L.nx * L.ny * L.nz, g, P);
+    return hipGetLastError();
+}
+
+long reduce_blocks(const ewpt_lattice &L)
+{
+    const dim3 gr = grid_of(L);
+    return (long)gr.x * gr.y * gr.z;
+}
+
+hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
+                           hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    const long n3 = (long)L.nx * L.ny * L.nz;
+    energy_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
+    finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 1u << 8, res);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
+                        hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    const long n3 = (long)L.nx * L.ny * L.nz;
+    gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
+    finish_reduce<<<8, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    return hipGetLastError();
+}
+
+}  // namespace ewpt

@@ -1,0 +1,34 @@
+// ewpt_internal.h -- declarations shared between the kernel files and the
+// C-ABI implementation of libewpt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/ewpt.h"
+#include "ewpt_point.h"
+#include "ewpt_stage.h"
+#include "ewpt_stencil.h"
+
+namespace ewpt {
+
+// ewpt_gather.hip
+hipError_t launch_stage_gather(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P,
+                               double dt, hipStream_t st);
+hipError_t launch_rhs_gather(const double *in, double *out, const ewpt_lattice &L, const Phys &P, hipStream_t st);
+long reduce_blocks(const ewpt_lattice &L);
+hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
+                           hipStream_t st);
+hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
+                        hipStream_t st);
+
+// ewpt_init.hip
+void mode_table(unsigned long long seed, int nmodes, int kmax, double amp, double mom_scale, double *t);
+hipError_t launch_fill_modes(double *y, const ewpt_lattice &L, const double *dmodes, int nmodes, double vev,
+                             hipStream_t st);
+
+inline Phys phys_of(const ewpt_params &p)
+{
+    return Phys{p.gw, p.gy, p.gp2, p.beta_w, p.beta_y, p.vev, p.lambda, p.jex, p.gamma};
+}
+
+}  // namespace ewpt

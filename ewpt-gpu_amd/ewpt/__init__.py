@@ -1,0 +1,11 @@
+"""ewpt -- host side of the MI355X RK4 path of the electroweak lattice evolution.
+
+Mirrors the driver surface of Teerthal/EWPT-GPU (gpu-kernels-rk4/main_rk4.jl
+run(), parameters.jl) over the C ABI of libewpt (include/ewpt.h).
+"""
+from ._lib import (EWPT_FP64, EWPT_MIXED, EwptError, EwptLattice, EwptParams, NCH, NK, load)
+from .lattice import Lattice, as_params
+from .parameters import Parameters, J_ext, damp
+
+__all__ = ["EWPT_FP64", "EWPT_MIXED", "EwptError", "EwptLattice", "EwptParams", "NCH", "NK", "load",
+           "Lattice", "as_params", "Parameters", "J_ext", "damp"]

@@ -1,0 +1,105 @@
+"""ctypes binding of libewpt (the HIP library, C ABI in include/ewpt.h).
+
+The library must be built (``__graft_entry__.build()``); importing this
+module on a machine without the built library raises instead of falling back
+to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libewpt.so")
+
+NCH = 36
+NK = 20
+
+EWPT_OK = 0
+EWPT_ERR_INVAL = -1
+EWPT_ERR_HIP = -2
+EWPT_ERR_OOM = -3
+EWPT_ERR_NONFINITE = -4
+EWPT_ERR_STATE = -5
+EWPT_ERR_UNSUPPORTED = -6
+
+EWPT_FP64 = 0
+EWPT_MIXED = 1
+
+# every symbol include/ewpt.h declares
+EXPORTS = (
+    "ewpt_create", "ewpt_destroy", "ewpt_set_state", "ewpt_get_state", "ewpt_rhs",
+    "ewpt_rhs_device", "ewpt_step", "ewpt_step_timed", "ewpt_energies", "ewpt_gauss",
+    "ewpt_set_variant", "ewpt_get_variant", "ewpt_device_bytes", "ewpt_strerror",
+    "ewpt_last_error", "ewpt_version", "ewpt_init_modes",
+)
+
+
+class EwptLattice(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dx", ctypes.c_double)]
+
+
+class EwptParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("gw", "gy", "gp2", "beta_w", "beta_y", "vev", "lambda_", "dt", "jex", "gamma")]
+
+
+class EwptError(RuntimeError):
+    def __init__(self, code: int, where: str, detail: str = ""):
+        self.code = code
+        msg = f"{where}: {strerror(code)} ({code})"
+        if detail:
+            msg += f": {detail}"
+        super().__init__(msg)
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libewpt.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libewpt.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+    lp = ctypes.POINTER(EwptLattice)
+    pp = ctypes.POINTER(EwptParams)
+    sig = {
+        "ewpt_create": [ctypes.POINTER(vp), lp, ip, ip],
+        "ewpt_destroy": [vp],
+        "ewpt_set_state": [vp, dp],
+        "ewpt_get_state": [vp, dp],
+        "ewpt_rhs": [vp, pp, dp, dp],
+        "ewpt_rhs_device": [lp, pp, vp, vp, vp],
+        "ewpt_step": [vp, pp, ip],
+        "ewpt_step_timed": [vp, pp, ip, dp, dp],
+        "ewpt_energies": [vp, pp, dp],
+        "ewpt_gauss": [vp, pp, dp],
+        "ewpt_set_variant": [vp, ip],
+        "ewpt_get_variant": [vp, ctypes.POINTER(ip)],
+        "ewpt_device_bytes": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
+        "ewpt_init_modes": [vp, ctypes.c_ulonglong, ip, ip, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.ewpt_strerror.argtypes = [ip]
+    L.ewpt_strerror.restype = ctypes.c_char_p
+    L.ewpt_last_error.argtypes = [vp]
+    L.ewpt_last_error.restype = ctypes.c_char_p
+    L.ewpt_version.argtypes = []
+    L.ewpt_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def strerror(code: int) -> str:
+    try:
+        return load().ewpt_strerror(code).decode()
+    except RuntimeError:
+        return f"error {code}"

@@ -1,0 +1,152 @@
+"""Host-side handle on one device lattice (one libewpt context).
+
+``Lattice`` owns the 36-channel state on the GPU and exposes the operations
+the reference performs on its CuArrays inside run()
+(gpu-kernels-rk4/main_rk4.jl:5174-7081): one RHS evaluation (the rk4_step!
+launch), full RK4 steps (the evolve loop body), and the energy / Gauss
+reductions.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import EwptError, EwptLattice, EwptParams, NCH
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def as_params(p) -> EwptParams:
+    """EwptParams from an EwptParams, a dict, or an object with the ten fields."""
+    if isinstance(p, EwptParams):
+        return p
+    names = ("gw", "gy", "gp2", "beta_w", "beta_y", "vev", "lambda_", "dt", "jex", "gamma")
+    get = (lambda k: p[k]) if isinstance(p, dict) else (lambda k: getattr(p, k))
+    return EwptParams(*[float(get(k)) for k in names])
+
+
+class Lattice:
+    """A periodic nx*ny*nz lattice of the electroweak fields on one GPU."""
+
+    def __init__(self, n, dx: float, precision: int = _lib.EWPT_FP64, device: int = 0):
+        self.nx, self.ny, self.nz = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
+        self.dx = float(dx)
+        self.precision = precision
+        self._L = _lib.load()
+        self._lat = EwptLattice(self.nx, self.ny, self.nz, self.dx)
+        h = ctypes.c_void_p()
+        rc = self._L.ewpt_create(ctypes.byref(h), ctypes.byref(self._lat), precision, device)
+        if rc != 0:
+            raise EwptError(rc, "ewpt_create")
+        self._h = h
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ewpt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- helpers
+    @property
+    def shape(self) -> tuple[int, int, int, int]:
+        """numpy shape of a state: (36, nz, ny, nx), x fastest."""
+        return (NCH, self.nz, self.ny, self.nx)
+
+    def _check(self, rc: int, where: str):
+        if rc != 0:
+            raise EwptError(rc, where, self._L.ewpt_last_error(self._h).decode())
+
+    def _host(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"state shape {a.shape} != {self.shape}")
+        return a
+
+    # ---- state
+    def set_state(self, state: np.ndarray):
+        a = self._host(state)
+        self._check(self._L.ewpt_set_state(self._h, _dptr(a)), "ewpt_set_state")
+
+    def init_modes(self, seed: int, nmodes: int = 4, kmax: int = 2, amp: float = 0.3, vev: float = 1.0,
+                   mom_scale: float = 1.0) -> np.ndarray:
+        """Seeded smooth state generated on the device; returns the mode table
+        (36, nmodes, 5) = kx, ky, kz, phase, amplitude."""
+        t = np.zeros((NCH, nmodes, 5))
+        self._check(self._L.ewpt_init_modes(self._h, ctypes.c_ulonglong(seed), nmodes, kmax, amp, vev, mom_scale,
+                                            _dptr(t)), "ewpt_init_modes")
+        return t
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        self._check(self._L.ewpt_get_state(self._h, _dptr(out)), "ewpt_get_state")
+        return out
+
+    # ---- the hot path
+    def rhs(self, state: np.ndarray, p) -> np.ndarray:
+        """One rk4_step! evaluation on host arrays (main_rk4.jl:52-661)."""
+        a = self._host(state)
+        out = np.empty(self.shape)
+        P = as_params(p)
+        self._check(self._L.ewpt_rhs(self._h, ctypes.byref(P), _dptr(a), _dptr(out)), "ewpt_rhs")
+        return out
+
+    def step(self, p, nsteps: int = 1):
+        """nsteps RK4 steps with fixed scalars (main_rk4.jl:6146-6441)."""
+        P = as_params(p)
+        self._check(self._L.ewpt_step(self._h, ctypes.byref(P), int(nsteps)), "ewpt_step")
+
+    def step_timed(self, p, nsteps: int):
+        """Like step(); returns (mean ms per launch of stages 1..4, total ms)."""
+        P = as_params(p)
+        st = (ctypes.c_double * 4)()
+        tot = ctypes.c_double()
+        self._check(self._L.ewpt_step_timed(self._h, ctypes.byref(P), int(nsteps), st, ctypes.byref(tot)),
+                    "ewpt_step_timed")
+        return list(st), tot.value
+
+    # ---- outputs
+    def energies(self, p) -> np.ndarray:
+        """13-column energy row (main_rk4.jl:6104-6116)."""
+        out = np.zeros(13)
+        P = as_params(p)
+        self._check(self._L.ewpt_energies(self._h, ctypes.byref(P), _dptr(out)), "ewpt_energies")
+        return out
+
+    def gauss(self, p) -> np.ndarray:
+        """Gauss-law residual norms G^1..3, G^Y, C^1..3, C^Y."""
+        out = np.zeros(8)
+        P = as_params(p)
+        self._check(self._L.ewpt_gauss(self._h, ctypes.byref(P), _dptr(out)), "ewpt_gauss")
+        return out
+
+    # ---- tuning
+    @property
+    def variant(self) -> int:
+        v = ctypes.c_int()
+        self._check(self._L.ewpt_get_variant(self._h, ctypes.byref(v)), "ewpt_get_variant")
+        return v.value
+
+    @variant.setter
+    def variant(self, v: int):
+        self._check(self._L.ewpt_set_variant(self._h, int(v)), "ewpt_set_variant")
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_ulonglong()
+        self._check(self._L.ewpt_device_bytes(self._h, ctypes.byref(b)), "ewpt_device_bytes")
+        return b.value

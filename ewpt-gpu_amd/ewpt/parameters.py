@@ -1,0 +1,109 @@
+"""Run parameters, mirroring gpu-kernels-rk4/parameters.jl.
+
+The reference parses 12 positional ARGS into module globals
+(parameters.jl:14-55) and derives the rest.  ``Parameters.from_args`` takes
+the same 12 strings in the same order::
+
+    1 gamma  2 no_bubbles  3 nte  4 dx  5 dt-divisor  6 N  7 gp2  8 nsnaps
+    9 T      10 beta_Y     11 beta_W  12 seed multiplier
+
+Hard-coded globals keep the reference values (gw=0.65, sin^2 theta_W=0.22,
+lambda=0.129, vev=1, t_sat=2).  ``gy`` defaults to the reference's
+derived value gw*tan(asin(sqrt(0.22))) (parameters.jl:16-18) and can be
+overridden (the benchmark configurations quote 0.35).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+from ._lib import EwptParams
+
+
+@dataclass
+class Parameters:
+    N: int = 64
+    dx: float = 0.25
+    dt_div: float = 10.0
+    nte: int = 200
+    gamma_inp: float = 0.0
+    no_bubbles: int = 10
+    gp2: float = 0.99
+    nsnaps: int = 10
+    T: float = 0.5
+    beta_Y: float = 0.0
+    beta_W: float = 0.0
+    seed_mult: int = 1
+    gw: float = 0.65
+    sin2_thetaW: float = 0.22
+    gy: float = field(default=float("nan"))
+    vev: float = 1.0
+    lambda_: float = 0.129
+    t_sat: float = 2.0
+
+    def __post_init__(self):
+        if math.isnan(self.gy):
+            self.gy = self.gw * math.tan(self.Wangle)   # parameters.jl:16-18
+
+    # ---- derived quantities (parameters.jl:17-98)
+    @property
+    def Wangle(self) -> float:
+        return math.asin(math.sqrt(self.sin2_thetaW))
+
+    @property
+    def dt(self) -> float:
+        return self.dx / self.dt_div                   # parameters.jl:23
+
+    @property
+    def mH(self) -> float:
+        return 2.0 * math.sqrt(self.lambda_) * self.vev  # parameters.jl:24
+
+    @property
+    def meff_sq(self) -> float:
+        return 8119.43 / (174.0 ** 2)                   # parameters.jl:37
+
+    @property
+    def dsnaps(self) -> int:
+        return int(math.floor(self.nte / self.nsnaps))  # parameters.jl:46
+
+    @property
+    def base_seed(self) -> int:
+        return 123456 * self.seed_mult                  # parameters.jl:53
+
+    @property
+    def bub_diam(self) -> int:
+        # floor(int_prec, 30*0.1 ÷ dx): ÷ truncates (parameters.jl:65)
+        return int(math.trunc((30 * 0.1) / self.dx))
+
+    @property
+    def Vol(self) -> float:
+        return self.dx ** 3                             # parameters.jl:82
+
+    @property
+    def spec_cut(self) -> tuple[int, int, int]:
+        return (self.N // 4,) * 3                       # parameters.jl:78
+
+    @classmethod
+    def from_args(cls, args: list[str], **over) -> "Parameters":
+        """The 12 positional ARGS of parameters.jl:14-55."""
+        if len(args) < 12:
+            raise ValueError(f"expected 12 positional arguments, got {len(args)}")
+        p = cls(gamma_inp=float(args[0]), no_bubbles=int(args[1]), nte=int(args[2]), dx=float(args[3]),
+                dt_div=float(args[4]), N=int(args[5]), gp2=float(args[6]), nsnaps=int(args[7]),
+                T=float(args[8]), beta_Y=float(args[9]), beta_W=float(args[10]), seed_mult=int(args[11]))
+        return replace(p, **over) if over else p
+
+    def step_params(self, jex: float = 0.0, gamma: float = 0.0) -> EwptParams:
+        """Scalars of one rk4_step! launch (main_rk4.jl:74) for this step."""
+        return EwptParams(self.gw, self.gy, self.gp2, self.beta_W, self.beta_Y, self.vev,
+                          self.lambda_, self.dt, jex, gamma)
+
+
+def J_ext(it: int, dt: float, mH: float, t_sat: float, thermal_init: bool = False) -> float:
+    """External Higgs mass term (main_rk4.jl:5154-5162)."""
+    return mH ** 2 if (it * dt * mH <= t_sat and thermal_init) else 0.0
+
+
+def damp(it: int, dt: float, mH: float, t_sat: float, gamma_inp: float, thermal_init: bool = False) -> float:
+    """Higgs damping switch (main_rk4.jl:5164-5172)."""
+    return 0.0 if (it * dt * mH < t_sat and thermal_init) else gamma_inp

@@ -1,0 +1,136 @@
+/*
+ * ewpt.h -- C ABI of libewpt, the MI355X implementation of the RK4 time step
+ * of the SU(2)xU(1) Higgs-gauge lattice evolution of Teerthal/EWPT-GPU.
+ *
+ * Plain C types only; callable through Julia `ccall`, Python `ctypes`, or C.
+ * Every entry point returns 0 on success or a negative EWPT_ERR_* code; none
+ * throws or aborts.  A context owns its device memory and one HIP stream; the
+ * caller owns every host buffer.  Calls are synchronous (they return after the
+ * device work finished) unless the name says otherwise.  A context is not
+ * re-entrant; distinct contexts may be used from distinct threads.
+ *
+ * State layout ("36-channel state"): 36 channels, each an nx*ny*nz FP64 array
+ * with x fastest (element (i,j,k) at i + nx*(j + ny*k)), channels in the
+ * argument order of rk4_step! (gpu-kernels-rk4/main_rk4.jl:52-74):
+ *    0-3   phi_1..phi_4                 20-23 dphi_a/dt   (pi)
+ *    4-12  W^a_i, a=1..3 outer, i=x,y,z  24-32 dW^a_i/dt  (E^a_i)
+ *    13-15 Y_x, Y_y, Y_z                 33-35 dY_i/dt    (E^Y_i)
+ *    16-18 Gamma^1..3, 19 Sigma
+ * This is the column-major Julia Array{Float64,3} of the reference, seen
+ * through a C pointer.
+ */
+#ifndef EWPT_H
+#define EWPT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EWPT_NCH 36
+
+/* error codes */
+#define EWPT_OK                0
+#define EWPT_ERR_INVAL        -1  /* bad argument (null pointer, size, enum) */
+#define EWPT_ERR_HIP          -2  /* a HIP runtime call failed */
+#define EWPT_ERR_OOM          -3  /* device allocation failed */
+#define EWPT_ERR_NONFINITE    -4  /* a reduction met NaN/Inf in the state */
+#define EWPT_ERR_STATE        -5  /* call made in the wrong order (no state) */
+#define EWPT_ERR_UNSUPPORTED  -6  /* valid request this build cannot serve */
+
+/* precision of a context */
+#define EWPT_FP64   0   /* FP64 storage and arithmetic */
+#define EWPT_MIXED  1   /* FP32 storage/stencils, compensated RK4 sum, FP64 reductions */
+
+typedef struct ewpt_ctx ewpt_ctx;
+
+/* Lattice: periodic nx*ny*nz, spacing dx (parameters.jl:14, 45). */
+typedef struct {
+    int nx, ny, nz;
+    double dx;
+} ewpt_lattice;
+
+/* Per-step physics scalars: the ten trailing scalars of rk4_step!
+ * (main_rk4.jl:74: gw,gy,gp2,beta_W,beta_Y,vev,lambda,dx,Jex,gamma) with dx
+ * moved into ewpt_lattice and dt added (parameters.jl:46). */
+typedef struct {
+    double gw, gy, gp2, beta_w, beta_y, vev, lambda, dt, jex, gamma;
+} ewpt_params;
+
+/* Replaces the CUDA.zeros allocation block of run() (main_rk4.jl:5176-5299).
+ * nx,ny,nz >= 8.  device = HIP ordinal. */
+int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int device);
+int ewpt_destroy(ewpt_ctx *ctx);
+
+/* Copy the 36-channel state in / out (host buffers of 36*nx*ny*nz doubles).
+ * Replaces the IC upload and the Array(...) downloads of run()
+ * (main_rk4.jl:5640-5698 write into, 6094-6103 read from the state). */
+int ewpt_set_state(ewpt_ctx *ctx, const double *host_state);
+int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
+
+/* Fill the context state on the device with a seeded smooth field: channel c
+ * is the sum of nmodes cosines with wave numbers |k_x|,|k_y|,|k_z| <= kmax,
+ * phases uniform in [0,2pi) and amplitudes uniform in [-amp,amp]/nmodes,
+ * plus vev on phi_4 (channel 3); channels 20..35 (momenta) are scaled by
+ * mom_scale.  The mode table comes from splitmix64(seed); the same table is
+ * returned in modes_out (if not NULL, 36*nmodes*5 doubles: kx,ky,kz,phase,amp)
+ * so that a CPU can rebuild the identical field.  Synthetic input for tests
+ * and benchmarks at sizes where a host upload is impractical. */
+int ewpt_init_modes(ewpt_ctx *ctx, unsigned long long seed, int nmodes, int kmax, double amp,
+                    double vev, double mom_scale, double *modes_out);
+
+/* One RHS evaluation, host buffers: the @cuda rk4_step!(...) launch
+ * (main_rk4.jl:52-661; launch 6155).  in = 36-channel stage state, out =
+ * the 36 k/kt arrays in the same channel order (0-15 k = momenta copies,
+ * 16-19 k_Gamma/k_Sigma, 20-35 kt). */
+int ewpt_rhs(ewpt_ctx *ctx, const ewpt_params *p, const double *host_in, double *host_out);
+
+/* Same on device buffers (FP64, 36*nx*ny*nz each, distinct), enqueued on
+ * `stream` (a hipStream_t, NULL = default stream); asynchronous. */
+int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p,
+                    const double *dev_in, double *dev_out, void *stream);
+
+/* nsteps full RK4 steps of the context state with fixed scalars: the
+ * `for it in 1:nte` body of main_rk4.jl:6146-6441 (4 stage launches + the
+ * broadcast combinations), with the stage combinations fused into the
+ * stage kernels. */
+int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
+
+/* Like ewpt_step, and reports mean device time per launch (ms) of each
+ * stage kernel, measured with HIP events on the context stream:
+ * stage_ms[0..3] = stages 1..4.  *total_ms = event time of the whole run. */
+int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps,
+                    double stage_ms[4], double *total_ms);
+
+/* Energy table in the reference's column order (main_rk4.jl:6104-6116):
+ * out[0..6] = PE, KE, GE, EEW, MEW, EEY, MEY (sum * dx^3, FP64),
+ * out[7], out[8] = EM magnetic energies (B, B_2 normalisations; 0 until the
+ * B-field kernel runs), out[9] = min|phi|, out[10] = 0, out[11] = mean|phi|,
+ * out[12] = 0 (helicity not computed).
+ * Returns EWPT_ERR_NONFINITE (after filling out) if any density is NaN/Inf. */
+int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13]);
+
+/* Gauss-law residual norms, sqrt(sum r^2 dx^3):
+ * out[0..2] G^a = div E^a - rho^a, out[3] G^Y = div E^Y - rho^Y,
+ * out[4..6] C^a = Gamma^a - div W^a, out[7] C^Y = Sigma - div Y,
+ * where rho is the charge that k_Gamma/k_Sigma weight by gp2
+ * (main_rk4.jl:603-657).  Returns EWPT_ERR_NONFINITE if any is NaN/Inf. */
+int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
+
+/* Kernel variant of the stage launches (0 = default).  Variants compute the
+ * same result; they exist for the autotuner and for measurements. */
+int ewpt_set_variant(ewpt_ctx *ctx, int variant);
+int ewpt_get_variant(ewpt_ctx *ctx, int *variant);
+
+/* Device bytes the context holds. */
+int ewpt_device_bytes(ewpt_ctx *ctx, unsigned long long *bytes);
+
+const char *ewpt_strerror(int code);
+/* Last error message of this context ("" if none). */
+const char *ewpt_last_error(ewpt_ctx *ctx);
+/* Library version string. */
+const char *ewpt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EWPT_H */
