@@ -108,10 +108,22 @@ StageBufs<double> bufs_for(ewpt_ctx *c, int stage)
 
 int set_device(ewpt_ctx *c) { HIPCHK(c, hipSetDevice(c->device), "hipSetDevice"); return EWPT_OK; }
 
+// variant actually launched: AUTO resolves to the 16x8x4 brick when the
+// lattice tiles by it, else to the gather kernel
+int resolved_variant(const ewpt_ctx *c)
+{
+    if (c->variant == VARIANT_AUTO) return brick_supported(1, c->lat) ? 1 : VARIANT_GATHER;
+    return c->variant;
+}
+
 int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
 {
     const Phys P = phys_of(*p);
-    HIPCHK(c, launch_stage_gather(stage, bufs_for(c, stage), c->lat, P, p->dt, c->stream), "stage launch");
+    const int v = resolved_variant(c);
+    if (v == VARIANT_GATHER)
+        HIPCHK(c, launch_stage_gather(stage, bufs_for(c, stage), c->lat, P, p->dt, c->stream), "stage launch");
+    else
+        HIPCHK(c, launch_stage_brick(v, stage, bufs_for(c, stage), c->lat, P, p->dt, c->stream), "stage launch");
     if (stage == 4) std::swap(c->y, c->ynext);
     return EWPT_OK;
 }
@@ -349,7 +361,11 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8])
 
 int ewpt_set_variant(ewpt_ctx *ctx, int variant)
 {
-    if (!ctx || variant < 0 || variant > 0) return EWPT_ERR_INVAL;
+    if (!ctx) return EWPT_ERR_INVAL;
+    if (variant != VARIANT_AUTO && variant != VARIANT_GATHER && (variant < 1 || variant > brick_variant_count()))
+        return EWPT_ERR_INVAL;
+    if (variant >= 1 && variant <= brick_variant_count() && !brick_supported(variant, ctx->lat))
+        return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
     ctx->variant = variant;
     return EWPT_OK;
 }

@@ -21,6 +21,16 @@ hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P
 hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
                         hipStream_t st);
 
+// ewpt_brick.hip -- LDS-tiled fused stage kernels, variants 1..5
+int brick_variant_count();
+bool brick_supported(int variant, const ewpt_lattice &L);
+hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
+                              const Phys &P, double dt, hipStream_t st);
+
+// stage kernel variants selectable through ewpt_set_variant
+constexpr int VARIANT_AUTO = 0;
+constexpr int VARIANT_GATHER = 9;
+
 // ewpt_init.hip
 void mode_table(unsigned long long seed, int nmodes, int kmax, double amp, double mom_scale, double *t);
 hipError_t launch_fill_modes(double *y, const ewpt_lattice &L, const double *dmodes, int nmodes, double vev,

@@ -116,8 +116,14 @@ int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13]);
  * (main_rk4.jl:603-657).  Returns EWPT_ERR_NONFINITE if any is NaN/Inf. */
 int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 
-/* Kernel variant of the stage launches (0 = default).  Variants compute the
- * same result; they exist for the autotuner and for measurements. */
+/* Kernel variant of the stage launches.  All variants compute the same
+ * result; they exist for the autotuner and for measurements.
+ *   0 = automatic (the 16x8x4 LDS brick when the lattice tiles by it, else 9)
+ *   1..5 = LDS-brick stage kernel: 1 = 16x8x4, 2 = 8x8x8, 3 = 32x4x4,
+ *          4 = 16x16x2 (XCD-aware block order), 5 = 16x8x4 (linear order)
+ *   9 = gather kernel (one site per thread, any lattice size)
+ * Setting a brick variant that does not tile the lattice returns
+ * EWPT_ERR_UNSUPPORTED. */
 int ewpt_set_variant(ewpt_ctx *ctx, int variant);
 int ewpt_get_variant(ewpt_ctx *ctx, int *variant);
 

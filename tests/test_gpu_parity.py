@@ -107,3 +107,40 @@ def test_init_modes_matches_cpu_rebuild():
     ref = fields.modes_state(shape, table, vev=1.0)
     assert np.abs(y - ref).max() < 1e-13
     assert np.all(np.abs(table[:, :, :3]) <= 2)
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 9])
+def test_stage_variants_match_oracle(variant):
+    shape = (32, 32, 32)
+    print(f"variant {variant} shape={shape}", flush=True)
+    p = fields.params(beta_w=0.2, beta_y=0.1, gamma=0.3, jex=0.5)
+    s = fields.smooth_state(shape, seed=12, amp=0.5)
+    with ewpt.Lattice(shape, DX) as lat:
+        lat.variant = variant
+        lat.set_state(s)
+        lat.step(p, 3)
+        y = lat.get_state()
+    r = cref.rk4(s, p, DX, 3)
+    err = rel_l2_per_channel(y, r)
+    assert err.max() < 1e-12, (variant, err.argmax(), err.max())
+
+
+def test_brick_nonuniform_lattice():
+    shape = (48, 16, 24)      # 3x2x6 bricks of 16x8x4: 36 bricks, not a multiple of 8
+    p = fields.params(gamma=0.1)
+    s = fields.smooth_state(shape, seed=13)
+    with ewpt.Lattice(shape, DX) as lat:
+        assert lat.variant == 0
+        lat.set_state(s)
+        lat.step(p, 2)
+        y = lat.get_state()
+    r = cref.rk4(s, p, DX, 2)
+    assert rel_l2_per_channel(y, r).max() < 1e-12
+
+
+def test_variant_rejected_when_not_tiling():
+    with ewpt.Lattice((24, 24, 24), DX) as lat:
+        with pytest.raises(ewpt.EwptError) as ei:
+            lat.variant = 1          # 24 % 16 != 0
+        assert ei.value.code == ewpt._lib.EWPT_ERR_UNSUPPORTED
+        lat.variant = 2              # 8x8x8 tiles 24^3

@@ -1,0 +1,315 @@
+"""Pins for the CPU oracle (no GPU).
+
+The reference ships no tests, fixtures or golden vectors, and its Julia +
+CUDA.jl code cannot run here, so the oracle (oracle/ewpt_oracle.c, a literal
+transcription of gpu-kernels-rk4/main_rk4.jl:52-661 and 6146-6441) is pinned
+by checks a transcription mistake would fail:
+
+* an independent restatement in generator form (oracle/rhs_np.py) agrees to
+  rounding with every term switched on;
+* the stencils reproduce their closed-form Fourier symbols
+  (diff_scheme.jl:157-311) through the full RHS;
+* the decoupled limit gw = gy = 0 matches an FFT (spectral) evaluation of the
+  reduced equations;
+* the vacuum phi = (0,0,0,v) is static;
+* constant SU(2) x U(1) rotations commute with the RHS (pins every sign and
+  index of the non-abelian and Higgs-current terms);
+* lattice translations commute with the RHS bit for bit;
+* linearised Higgs waves have omega^2 = K(theta)^2 + m_H^2, m_H = 2 sqrt(lambda) v
+  (parameters.jl:24);
+* the RK4 loop matches a numpy loop in the reference's order bit for bit, and
+  converges at fourth order on a linear mode;
+* energy (main_rk4.jl:1334-2301) is conserved by the continuum dynamics and
+  drifts only at the discretisation level; the Gauss residual stays at that
+  level from pi = E = 0.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle import cref, fields, rhs_np
+from oracle.rhs_np import T, TY
+
+DX = 0.25
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def per_channel_rel(a, b):
+    return np.array([rel(a[c], b[c]) if np.linalg.norm(b[c]) > 0 else np.abs(a[c]).max() for c in range(36)])
+
+
+# ------------------------------------------------------------ restatements
+
+@pytest.mark.parametrize("shape,seed", [((8, 10, 12), 1), ((12, 12, 12), 2)])
+def test_c_matches_generator_form_all_terms(oracle_lib, shape, seed):
+    p = fields.params(beta_w=0.3, beta_y=0.2, gamma=0.1, jex=0.4, gp2=0.7)
+    s = fields.noise_state(shape, seed=seed)
+    a = cref.rhs(s, p, DX)
+    b = rhs_np.rhs(s, p, DX)
+    assert per_channel_rel(a, b).max() < 1e-14
+    np.testing.assert_allclose(cref.energies(s, p, DX)[[0, 1, 2, 3, 4, 5, 6, 9, 11]],
+                               rhs_np.energies(s, p, DX)[[0, 1, 2, 3, 4, 5, 6, 9, 11]], rtol=1e-13)
+    np.testing.assert_allclose(cref.gauss(s, p, DX), rhs_np.gauss(s, p, DX), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- stencils
+
+def _mode(shape, m, phase=0.3):
+    nx, ny, nz = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    th = [2 * np.pi * m[0] / nx, 2 * np.pi * m[1] / ny, 2 * np.pi * m[2] / nz]
+    arg = th[0] * x + th[1] * y + th[2] * z + phase
+    return th, arg
+
+
+def D1(th):
+    return (90 * np.sin(th) - 18 * np.sin(2 * th) + 2 * np.sin(3 * th)) / (60 * DX)
+
+
+def D2(th):
+    return (2 * np.cos(3 * th) / 90 - 2 * 0.15 * np.cos(2 * th) + 3 * np.cos(th) - 49 / 18) / DX ** 2
+
+
+@pytest.mark.parametrize("m", [(1, 0, 0), (0, 2, 0), (0, 0, 3), (1, 2, 3), (5, 1, 7)])
+def test_stencil_symbols(oracle_lib, m):
+    shape = (16, 12, 20)
+    th, arg = _mode(shape, m)
+    p = fields.params(gw=0.0, gy=0.0, gp2=0.5)
+    s = np.zeros((36,) + arg.shape)
+    s[3] = 1.0                      # vacuum Higgs
+    s[16] = np.sin(arg)             # Gamma^1  -> kt_W^1_i = -d_i Gamma^1
+    s[4] = np.sin(arg)              # W^1_x    -> kt_W^1_x also gets lap W^1_x
+    s[24] = np.sin(arg)             # E^1_x    -> k_Gamma^1 = (1-gp2) d_x E^1_x
+    out = cref.rhs(s, p, DX)
+    lap = sum(D2(t) for t in th)
+    np.testing.assert_allclose(out[24], lap * np.sin(arg) - D1(th[0]) * np.cos(arg), atol=1e-12)
+    np.testing.assert_allclose(out[25], -D1(th[1]) * np.cos(arg), atol=1e-12)
+    np.testing.assert_allclose(out[26], -D1(th[2]) * np.cos(arg), atol=1e-12)
+    np.testing.assert_allclose(out[16], 0.5 * D1(th[0]) * np.cos(arg), atol=1e-12)
+
+
+def test_translation_is_bitwise(oracle_lib):
+    p = fields.params(beta_w=0.2, beta_y=0.1, gamma=0.1, jex=0.3)
+    s = fields.noise_state((12, 10, 8), seed=4)
+    out = cref.rhs(s, p, DX)
+    sh = (3, -2, 5)
+    s2 = np.roll(s, sh, axis=(1, 2, 3))
+    out2 = cref.rhs(np.ascontiguousarray(s2), p, DX)
+    assert np.array_equal(out2, np.roll(out, sh, axis=(1, 2, 3)))
+
+
+# ------------------------------------------------------------- invariants
+
+def test_vacuum_is_static(oracle_lib):
+    p = fields.params(gamma=0.2, beta_w=0.3, beta_y=0.4)
+    s = np.zeros((36, 10, 10, 10))
+    s[3] = p["vev"]
+    out = cref.rhs(s, p, DX)
+    # the Laplacian of a constant is not exactly 0 in floating point
+    assert np.abs(out).max() < 1e-13
+
+
+def test_decoupled_limit_matches_spectral(oracle_lib):
+    """gw = gy = 0: four real phi^4 fields plus free gauge fields."""
+    shape = (12, 10, 14)
+    p = fields.params(gw=0.0, gy=0.0, gp2=0.6, jex=0.3, gamma=0.0)
+    s = fields.smooth_state(shape, seed=5, amp=0.5, kmax=3)
+    out = cref.rhs(s, p, DX)
+    nx, ny, nz = shape
+    kx = 2 * np.pi * np.fft.fftfreq(nx)
+    ky = 2 * np.pi * np.fft.fftfreq(ny)
+    kz = 2 * np.pi * np.fft.fftfreq(nz)
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    sym1 = [1j * D1(KX), 1j * D1(KY), 1j * D1(KZ)]
+    sym2 = D2(KX) + D2(KY) + D2(KZ)
+
+    def d1(f, i):
+        return np.real(np.fft.ifftn(sym1[i] * np.fft.fftn(f)))
+
+    def lap(f):
+        return np.real(np.fft.ifftn(sym2 * np.fft.fftn(f)))
+
+    phi = s[0:4]
+    r2 = np.sum(phi ** 2, axis=0)
+    for b in range(4):
+        ref = lap(phi[b]) - 2 * p["lambda_"] * (r2 - 1.0) * phi[b] - 0.5 * p["jex"] * phi[b]
+        assert rel(out[20 + b], ref) < 1e-12
+    for a in range(3):
+        for i in range(3):
+            ref = lap(s[4 + 3 * a + i]) - d1(s[16 + a], i)
+            assert rel(out[24 + 3 * a + i], ref) < 1e-12
+        ref = (1 - p["gp2"]) * sum(d1(s[24 + 3 * a + i], i) for i in range(3))
+        assert rel(out[16 + a], ref) < 1e-12
+    for i in range(3):
+        assert rel(out[33 + i], lap(s[13 + i]) - d1(s[19], i)) < 1e-12
+    assert rel(out[19], (1 - p["gp2"]) * sum(d1(s[33 + i], i) for i in range(3))) < 1e-12
+    assert np.array_equal(out[:16], s[20:36])
+
+
+def _rotate(s, U, R, V):
+    """Apply a constant SU(2) x U(1) transformation to a state.
+
+    Higgs-type channels (phi, pi) transform with V @ U, adjoint channels
+    (W, Gamma, E) with R; U(1) fields are invariant."""
+    s2 = s.copy()
+    for base in (0, 20):
+        s2[base:base + 4] = np.einsum("bc,c...->b...", V @ U, s[base:base + 4])
+    for base, stride in ((4, 3), (24, 3)):
+        for i in range(3):
+            idx = [base + stride * a + i for a in range(3)]
+            s2[idx] = np.einsum("ab,b...->a...", R, s[idx])
+    s2[16:19] = np.einsum("ab,b...->a...", R, s[16:19])
+    return s2
+
+
+@pytest.mark.parametrize("seed", [6, 7])
+def test_global_gauge_covariance(oracle_lib, seed):
+    """RHS(g.s) = g.RHS(s) for constant g in SU(2) x U(1).
+
+    [T_a, T_b] = 2 eps_abc T_c and T_Y commutes with every T_a, so
+    U = exp(alpha n.T) with adjoint matrix R_ab = <T_a, U T_b U^-1>/<T_a,T_a>
+    and V = exp(beta T_Y).  gamma = 0 and beta_W = 0 here: the reference's
+    gamma damping acts per component and its beta_W bracket uses E^1 in the
+    a = 2, 3 rows (main_rk4.jl:349-352), neither of which is covariant."""
+    rng = np.random.default_rng(seed)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    U = expm(0.7 * np.einsum("a,abc->bc", n, T))
+    V = expm(0.4 * TY)
+    R = np.array([[np.sum(T[a] * (U @ T[b] @ U.T)) / np.sum(T[a] * T[a]) for b in range(3)] for a in range(3)])
+    assert np.abs(R @ R.T - np.eye(3)).max() < 1e-14
+    p = fields.params(gp2=0.8, jex=0.2, beta_y=0.3)
+    s = fields.noise_state((10, 8, 12), seed=seed)
+    out = cref.rhs(s, p, DX)
+    out_rot = cref.rhs(np.ascontiguousarray(_rotate(s, U, R, V)), p, DX)
+    rot_out = _rotate(out, U, R, V)
+    assert per_channel_rel(out_rot, rot_out).max() < 1e-13
+
+
+def test_gauge_covariance_detects_a_planted_sign_error():
+    """The covariance check fails for a plausible transcription slip: the
+    colour indices of Gamma swapped in the gw*(W x Gamma) term of kt_W
+    (planted in a copy of the numpy restatement, not in the oracle)."""
+    rng = np.random.default_rng(3)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    U = expm(0.9 * np.einsum("a,abc->bc", n, T))
+    R = np.array([[np.sum(T[a] * (U @ T[b] @ U.T)) / np.sum(T[a] * T[a]) for b in range(3)] for a in range(3)])
+    V = np.eye(4)
+    p = fields.params()
+    s = fields.noise_state((8, 8, 8), seed=9)
+
+    def bad_rhs(st):
+        o = rhs_np.rhs(st, p, DX)
+        W = st[4:13].reshape((3, 3) + st.shape[1:])
+        G = st[16:19]
+        WxG = np.einsum("abc,bi...,c...->ai...", rhs_np.EPS, W, G)
+        WxG_slip = np.einsum("abc,bi...,c...->ai...", rhs_np.EPS, W, G[[1, 0, 2]])
+        o[24:33] += p["gw"] * (WxG - WxG_slip).reshape((9,) + st.shape[1:])
+        return o
+
+    good = per_channel_rel(rhs_np.rhs(_rotate(s, U, R, V), p, DX), _rotate(rhs_np.rhs(s, p, DX), U, R, V))
+    bad = per_channel_rel(bad_rhs(_rotate(s, U, R, V)), _rotate(bad_rhs(s), U, R, V))
+    assert good.max() < 1e-13
+    assert bad.max() > 1e-6          # ten orders above rounding
+
+
+def test_higgs_mass_and_dispersion(oracle_lib):
+    shape = (16, 16, 16)
+    m = (1, 2, 0)
+    th, arg = _mode(shape, m)
+    eps = 1e-7
+    p = fields.params(gw=0.0, gy=0.0)
+    s = np.zeros((36,) + arg.shape)
+    s[3] = p["vev"] + eps * np.cos(arg)
+    out = cref.rhs(s, p, DX)
+    K2 = -sum(D2(t) for t in th)
+    mH2 = (2 * np.sqrt(p["lambda_"]) * p["vev"]) ** 2
+    ratio = out[23] / (-(K2 + mH2) * eps * np.cos(arg))
+    mask = np.abs(np.cos(arg)) > 0.1
+    assert np.abs(ratio[mask] - 1).max() < 1e-5
+
+
+# --------------------------------------------------------------- RK4 loop
+
+def test_rk4_loop_bitwise_against_numpy_loop(oracle_lib):
+    p = fields.params(gamma=0.1, jex=0.2)
+    s = fields.smooth_state((8, 10, 12), seed=8)
+    a = cref.rk4(s, p, DX, 3)
+    b = rhs_np.rk4(s, p, DX, 3, rhs_fn=lambda u: cref.rhs(np.ascontiguousarray(u), p, DX))
+    assert np.array_equal(a, b)
+
+
+def test_rk4_fourth_order_on_linear_mode(oracle_lib):
+    """Free field (gw = gy = lambda = 0): phi_1 = cos(omega t) cos(k.x) with
+    omega = K(theta), the semi-discrete frequency."""
+    shape = (16, 8, 8)
+    th, arg = _mode(shape, (2, 0, 0), phase=0.0)
+    omega = np.sqrt(-sum(D2(t) for t in th))
+    s0 = np.zeros((36,) + arg.shape)
+    s0[0] = np.cos(arg)
+    T_end = 0.6     # omega*T_end away from a multiple of pi, so the phase error shows
+    errs = []
+    for nsteps in (8, 16, 32):
+        p = fields.params(gw=0.0, gy=0.0, lambda_=0.0, vev=0.0, dt=T_end / nsteps)
+        y = cref.rk4(s0, p, DX, nsteps)
+        errs.append(np.abs(y[0] - np.cos(omega * T_end) * np.cos(arg)).max())
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 14 < r1 < 18 and 14 < r2 < 18, errs
+
+
+def _constraint_state(shape):
+    s = fields.smooth_state(shape, seed=10, amp=0.3, kmax=1, momenta=False)
+    # start on the constraint surface: Gamma = div W, Sigma = div Y
+    for a in range(3):
+        s[16 + a] = sum(rhs_np.d1(s[4 + 3 * a + i], i, DX) for i in range(3))
+    s[19] = sum(rhs_np.d1(s[13 + i], i, DX) for i in range(3))
+    return s
+
+
+def test_energy_and_gauss_behaviour(oracle_lib):
+    """Smooth fields, gamma = Jex = beta = 0, pi = E = 0: total energy drifts
+    only at the discretisation level (~3e-5 here) and the Gauss residuals
+    stay ~1e-3; the planted-error test below shows what a slip does."""
+    p = fields.params(dt=0.01)
+    s = _constraint_state((16, 16, 16))
+    e0 = cref.energies(s, p, DX)[:7].sum()
+    g0 = cref.gauss(s, p, DX)
+    y = cref.rk4(s, p, DX, 100)
+    e1 = cref.energies(y, p, DX)[:7].sum()
+    g1 = cref.gauss(y, p, DX)
+    assert np.all(g0 < 1e-12)
+    assert abs(e1 - e0) / e0 < 1e-4, (e0, e1)
+    assert g1.max() < 1e-2, g1
+
+
+@pytest.mark.slow
+def test_gauss_residual_detects_planted_sign_errors():
+    """Flipping the sign of the Higgs current or of the gw*(W x Gamma) term of
+    kt_W (planted in copies of the numpy restatement) makes the Gauss
+    residual grow by two orders of magnitude in 60 steps."""
+    p = fields.params(dt=0.01)
+    s = _constraint_state((12, 12, 12))
+
+    def flip_current(u):
+        o = rhs_np.rhs(u, p, DX)
+        f, g = rhs_np.derived(u, p, DX)
+        cur = p["gw"] * np.einsum("b...,abc,ci...->ai...", f["phi"], T, g["D"])
+        o[24:33] -= 2 * cur.reshape((9,) + u.shape[1:])
+        return o
+
+    def flip_wxg(u):
+        o = rhs_np.rhs(u, p, DX)
+        W = u[4:13].reshape((3, 3) + u.shape[1:])
+        WxG = np.einsum("abc,bi...,c...->ai...", rhs_np.EPS, W, u[16:19])
+        o[24:33] += 2 * p["gw"] * WxG.reshape((9,) + u.shape[1:])
+        return o
+
+    good = cref.gauss(cref.rk4(s, p, DX, 60), p, DX).max()
+    assert good < 1e-2
+    for f in (flip_current, flip_wxg):
+        bad = cref.gauss(rhs_np.rk4(s, p, DX, 60, rhs_fn=f), p, DX).max()
+        assert bad > 20 * good, (f.__name__, bad, good)
