@@ -47,6 +47,26 @@ def cpu_baseline(n: int = 64, steps: int = 12):
             "sample": f"oracle/ewpt_oracle.c FP64 RK4, {n}^3 lattice x {steps} steps, {dt:.1f} s"}
 
 
+def stream_copy(nbytes: int = 4 << 30, reps: int = 10) -> int:
+    """Time a plain device-to-device copy of nbytes (read + write) with torch."""
+    import torch
+    n = nbytes // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"stream_copy_bytes_read": nbytes, "stream_copy_bytes_written": nbytes,
+                      "ms": ms, "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +76,12 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--stream-copy", action="store_true",
+                    help="only time a plain device copy (HBM ceiling / counter calibration)")
     a = ap.parse_args(argv)
+
+    if a.stream_copy:
+        return stream_copy()
 
     if a.gpus != 1:
         print(f"bench.py: this build runs the hot path on one GPU; --gpus {a.gpus} is not supported "

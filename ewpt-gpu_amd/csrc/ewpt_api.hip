@@ -108,12 +108,15 @@ StageBufs<double> bufs_for(ewpt_ctx *c, int stage)
 
 int set_device(ewpt_ctx *c) { HIPCHK(c, hipSetDevice(c->device), "hipSetDevice"); return EWPT_OK; }
 
-// variant actually launched: AUTO resolves to the 16x8x4 brick when the
-// lattice tiles by it, else to the gather kernel
+// variant actually launched: AUTO takes the first brick variant that tiles
+// the lattice from the measured ranking (512^3 FP64 on MI355X: 211, 210, 2),
+// else the gather kernel
 int resolved_variant(const ewpt_ctx *c)
 {
-    if (c->variant == VARIANT_AUTO) return brick_supported(1, c->lat) ? 1 : VARIANT_GATHER;
-    return c->variant;
+    if (c->variant != VARIANT_AUTO) return c->variant;
+    for (int v : {211, 210, 2})
+        if (brick_supported(v, c->lat)) return v;
+    return VARIANT_GATHER;
 }
 
 int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
@@ -362,9 +365,12 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8])
 int ewpt_set_variant(ewpt_ctx *ctx, int variant)
 {
     if (!ctx) return EWPT_ERR_INVAL;
-    if (variant != VARIANT_AUTO && variant != VARIANT_GATHER && (variant < 1 || variant > brick_variant_count()))
-        return EWPT_ERR_INVAL;
-    if (variant >= 1 && variant <= brick_variant_count() && !brick_supported(variant, ctx->lat))
+    if (variant == VARIANT_AUTO || variant == VARIANT_GATHER) {
+        ctx->variant = variant;
+        return EWPT_OK;
+    }
+    if (!brick_exists(variant)) return EWPT_ERR_INVAL;
+    if (!brick_supported(variant, ctx->lat))
         return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
     ctx->variant = variant;
     return EWPT_OK;

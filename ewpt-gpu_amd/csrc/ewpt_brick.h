@@ -1,0 +1,654 @@
+// ewpt_brick.h -- fused RK4 stage kernel, LDS-tiled (template; instantiated
+// by the ewpt_brick_v*.hip files, one variant per file).
+//
+// One workgroup owns a TX*TY*TZ brick of sites, one site per thread.  The 32
+// stencilled channels of the stage input are streamed one per pass through a
+// double-buffered LDS tile holding the brick plus the star-shaped halo the
+// radius-3 stencils of diff_scheme.jl:157-311 touch (the six 3-deep face
+// slabs; edges and corners are never read).  The halo of pass p+PF is in
+// flight in registers while pass p is computed.  The stage input y + c*k is
+// formed when a value is staged (ewpt_stage.h), so the reference's broadcast
+// temporaries (main_rk4.jl:6226-6242 ...) never exist.  Periodic wrap is done
+// on the global index; there is no halo exchange.
+//
+// The right-hand side of rk4_step! (main_rk4.jl:337-657) is linear in the
+// spatial derivatives once the 36 centre values are known.  Each channel's
+// derivatives are therefore folded into 20 output accumulators (plus the
+// magnetic-field accumulators B^a, B^Y and u_j = phi.d_j phi) as soon as they
+// are computed, instead of holding all 80 derivative values.  The formulas
+// are those of ewpt_point.h, distributed over the derivative terms.  The
+// passes run E, Gamma/Sigma, phi, Y, W so that centre values die early.
+// BETA = false drops the beta_W/beta_Y terms (the reference's default runs
+// have beta = 0), which removes u, B^Y and the E centre values from the
+// registers after their own passes.
+//
+// Stage S=4 also finishes the step: y' = k4*(dt/6) + n with
+// n = ((k1*(dt/6) + y) + k2*(dt/3)) + k3*(dt/3) (main_rk4.jl:6400-6437).  The
+// field channels of y' and the partial sums n of channels 16..35 are computed
+// first (n is parked in LDS until k4 is known).
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+#include "ewpt_internal.h"
+
+namespace ewpt {
+
+namespace {
+
+template <int TX, int TY, int TZ>
+struct BrickShape {
+    static constexpr int NT = TX * TY * TZ;
+    static constexpr int HX = TX + 6, HY = TY + 6, HZ = TZ + 6;
+    static constexpr int BOX = HX * HY * HZ;   // LDS values of the halo'd brick
+    static constexpr int BUF = BOX + 64;        // + one trash slot per lane for empty halo slots
+    static constexpr int NXF = 6 * TY * TZ;    // x-face halo elements
+    static constexpr int NYF = 6 * TX * TZ;
+    static constexpr int NZF = 6 * TX * TY;
+};
+
+struct BrickGeo {
+    int nx, ny, nz;
+    int nbx, nby, nbz;      // bricks per axis
+    int xcd_map;            // 1: remap block ids so each XCD walks a compact region
+    int order;              // brick enumeration, see brick_coords
+    double inv60dx, invdx2;
+};
+
+constexpr int NPASS = 32;
+
+// Pass schedule.  kind: 0 = E^a_i / E^Y_i (one direction), 1 = Gamma^a /
+// Sigma (gradient), 2 = phi_b, 3 = Y_i, 4 = W^a_i (gradient + Laplacian).
+__host__ __device__ constexpr int pass_kind(int p)
+{
+    return p < 12 ? 0 : p < 16 ? 1 : p < 20 ? 2 : p < 23 ? 3 : 4;
+}
+__host__ __device__ constexpr int pass_channel(int p)
+{
+    return p < 12 ? ((p % 4) < 3 ? CH_E + 3 * (p % 4) + p / 4 : CH_EY + p / 4)   // E^a_i, i = p/4
+         : p < 16 ? CH_GAM + (p - 12)                                             // Gamma^1..3, Sigma
+         : p < 20 ? CH_PHI + (p - 16)                                             // phi_1..4
+         : p < 23 ? CH_Y + (p - 20)                                               // Y_x, Y_y, Y_z
+         : CH_W + 3 * ((p - 23) % 3) + (p - 23) / 3;                              // W^a_i, component-major
+}
+// halo needed: 0 = full star, 1/2/3 = x/y/z faces only
+__host__ __device__ constexpr int pass_dir(int p) { return p < 12 ? 1 + p / 4 : 0; }
+
+__device__ __forceinline__ int wrapc(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Brick coordinates of enumeration index i.  Blocks with nearby indices run
+// at nearby times, so the order decides which halo re-reads find their data
+// still in L2 / the memory-side cache:
+//   0: x fastest, then y, then z
+//   1: x fastest, then z, then y
+//   2: x fastest, then (y,z) in 4x4 tiles of brick rows (z fastest inside)
+__device__ __forceinline__ void brick_coords(const BrickGeo &g, int i, int &bx, int &by, int &bz)
+{
+    bx = i % g.nbx;
+    const int r = i / g.nbx;   // brick-row index over (y, z)
+    if (g.order == 1) {
+        bz = r % g.nbz;
+        by = r / g.nbz;
+    } else if (g.order == 2 && g.nby % 4 == 0 && g.nbz % 4 == 0) {
+        const int tile = r / 16, in = r % 16;
+        const int tz = tile % (g.nbz / 4), ty = tile / (g.nbz / 4);
+        bz = tz * 4 + in % 4;
+        by = ty * 4 + in / 4;
+    } else {
+        by = r % g.nby;
+        bz = r / g.nby;
+    }
+}
+
+// box coordinates of halo element f of the faces selected by dir
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ void face_elem(int dir, int f, int &lx, int &ly, int &lz)
+{
+    using B = BrickShape<TX, TY, TZ>;
+    int d = dir;
+    if (dir == 0) {
+        if (f < B::NXF) d = 1;
+        else if (f < B::NXF + B::NYF) { d = 2; f -= B::NXF; }
+        else { d = 3; f -= B::NXF + B::NYF; }
+    }
+    if (d == 1) {
+        const int h = f % 6, r = f / 6;
+        lx = h < 3 ? h : TX + h;
+        ly = 3 + r % TY;
+        lz = 3 + r / TY;
+    } else if (d == 2) {
+        const int h = (f / TX) % 6;
+        lx = 3 + f % TX;
+        ly = h < 3 ? h : TY + h;
+        lz = 3 + f / (TX * 6);
+    } else {
+        const int h = f / (TX * TY);
+        lx = 3 + f % TX;
+        ly = 3 + (f / TX) % TY;
+        lz = h < 3 ? h : TZ + h;
+    }
+}
+
+template <int TX, int TY, int TZ>
+__host__ __device__ constexpr int face_count(int dir)
+{
+    using B = BrickShape<TX, TY, TZ>;
+    return dir == 0 ? B::NXF + B::NYF + B::NZF : dir == 1 ? B::NXF : dir == 2 ? B::NYF : B::NZF;
+}
+
+// column b of A_i = h(W1 T1 + W2 T2 + W3 T3) + hy Y T_Y applied to val, added
+// to acc (rows read off apply_gen in ewpt_point.h)
+template <int b, typename R>
+__device__ __forceinline__ void add_gen_col(R h, R W1, R W2, R W3, R hy, R Yv, R val, R acc[4])
+{
+    if constexpr (b == 0) {
+        acc[1] -= (h * W3 + hy * Yv) * val; acc[2] += h * W2 * val; acc[3] -= h * W1 * val;
+    } else if constexpr (b == 1) {
+        acc[0] += (h * W3 + hy * Yv) * val; acc[2] += h * W1 * val; acc[3] += h * W2 * val;
+    } else if constexpr (b == 2) {
+        acc[0] -= h * W2 * val; acc[1] -= h * W1 * val; acc[3] += (h * W3 - hy * Yv) * val;
+    } else {
+        acc[0] += h * W1 * val; acc[1] -= h * W2 * val; acc[2] += (hy * Yv - h * W3) * val;
+    }
+}
+
+// coefficient of v_b in phi.T_a v (a = 0,1,2) and phi.T_Y v
+template <int a, int b, typename R>
+__device__ __forceinline__ R phiT_coef(const R p[4])
+{
+    constexpr int tab[3][4][2] = {{{3, -1}, {2, 1}, {1, -1}, {0, 1}},
+                                  {{2, 1}, {3, 1}, {0, -1}, {1, -1}},
+                                  {{1, -1}, {0, 1}, {3, 1}, {2, -1}}};
+    return tab[a][b][1] > 0 ? p[tab[a][b][0]] : -p[tab[a][b][0]];
+}
+template <int b, typename R>
+__device__ __forceinline__ R phiTY_coef(const R p[4])
+{
+    constexpr int tab[4][2] = {{1, -1}, {0, 1}, {3, -1}, {2, 1}};
+    return tab[b][1] > 0 ? p[tab[b][0]] : -p[tab[b][0]];
+}
+
+// Load/store at a 32-bit byte offset from a (uniform) channel base pointer:
+// lets the compiler use the SGPR-base + VGPR-offset addressing form instead
+// of materialising one 64-bit address per channel.
+template <typename T>
+__device__ __forceinline__ T ldo(const T *base, unsigned boff)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *base, unsigned boff, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+// Workgroup barrier for LDS only.  __syncthreads() also fences global
+// memory, which makes every in-flight global load (the prefetched halo of
+// the next passes) complete before the barrier; here only the LDS accesses
+// are waited for.  The "memory" clobber keeps the compiler from moving
+// memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Opaque use/def of a register value: keeps the compiler from sinking the
+// arithmetic that produced it past the next barrier (which would keep every
+// staged LDS value alive across passes).
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(float &x) { asm volatile("" : "+v"(x)); }
+
+// sign of eps^{a b c} for the third index c
+__host__ __device__ constexpr int eps3(int a, int b) { return ((b - a + 3) % 3 == 1) ? 1 : -1; }
+
+// In-flight loads of one pass: the halo values (up to 3 arrays per element).
+template <typename T, int MAXF>
+struct PassRegs {
+    T v[MAXF][3];
+};
+
+// Prefetch depth of stage S from the profile PF: a single digit applies to
+// all stages, four digits d1 d2 d3 d4 give stages 1..4 (deeper where a
+// halo element is one array, shallower where it is three).
+__host__ __device__ constexpr int pf_of_stage(int PF, int S)
+{
+    return PF < 10 ? PF : (PF / (S == 1 ? 1000 : S == 2 ? 100 : S == 3 ? 10 : 1)) % 10;
+}
+
+// T: storage and arithmetic type (double here).  PF: prefetch profile
+// (pf_of_stage).  Two waves per SIMD are requested so that the compiler
+// keeps within 256 VGPRs also for 256-thread bricks (two workgroups per CU).
+template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T = double>
+__global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(2)))
+stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
+{
+    using B = BrickShape<TX, TY, TZ>;
+    using R = T;
+    // dynamic LDS: two staging buffers, then (stage 4) the 20 partial sums n
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    R *lds_base = reinterpret_cast<R *>(lds_raw);
+    R *lds[2] = {lds_base, lds_base + B::BUF};
+    R *nstash = lds_base + 2 * B::BUF;
+
+    // ---- brick of this workgroup
+    int bid = blockIdx.x;
+    const int nbrick = g.nbx * g.nby * g.nbz;
+    if (g.xcd_map) {
+        // consecutive ids go to different XCDs; give XCD x the contiguous range
+        // [x*nbrick/8, (x+1)*nbrick/8) so its resident workgroups are neighbours
+        const int per = nbrick >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    int bx, by, bz;
+    brick_coords(g, bid, bx, by, bz);
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
+    const int t = threadIdx.x;
+    const int tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
+    const long nxl = g.nx, nxy = (long)g.nx * g.ny;
+    const unsigned qb = (unsigned)((x0 + tx) + nxl * (y0 + ty) + nxy * (z0 + tz)) * (unsigned)sizeof(T);
+    auto Y = [&](int ch) { return b.y + (long)ch * b.n3; };
+    auto K = [&](const T *k, int m) { return k + (long)m * b.n3; };
+    const R half_dt = dt * R(0.5), d6 = dt / R(6), d3 = dt / R(3);
+
+    // ---- stage 4: the field channels of y' and the partial sums n of
+    // channels 16..35 (parked in LDS until k4 is known) need only centre
+    // values; they are done first, in two batches of loads.
+    if constexpr (S == 4) {
+#pragma unroll
+        for (int m = 0; m < NK; m++) {
+            R n = ldo(K(b.k1, m), qb) * d6 + ldo(Y(16 + m), qb);
+            n = ldo(K(b.k2, m), qb) * d3 + n;
+            nstash[m * B::NT + t] = ldo(K(b.k3, m), qb) * d3 + n;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int ch = 0; ch < 16; ch++) {
+            const R piv = ldo(Y(ch + 20), qb);
+            const R k2 = piv + ldo(K(b.k1, ch + 4), qb) * half_dt;
+            const R k3 = piv + ldo(K(b.k2, ch + 4), qb) * half_dt;
+            const R k4 = piv + ldo(K(b.k3, ch + 4), qb) * dt;
+            R n = piv * d6 + ldo(Y(ch), qb);
+            n = k2 * d3 + n;
+            n = k3 * d3 + n;
+            sto(b.ynew + (long)ch * b.n3, qb, k4 * d6 + n);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // ---- centre values of the stage input (ewpt_stage.h)
+    R c[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) {
+        const R yv = ldo(Y(ch), qb);
+        if constexpr (S == 1) {
+            c[ch] = yv;
+        } else if (ch < 16) {
+            const R piv = ldo(Y(ch + 20), qb);
+            if constexpr (S == 2) c[ch] = yv + piv * half_dt;
+            if constexpr (S == 3) c[ch] = yv + (piv + ldo(K(b.k1, ch + 4), qb) * half_dt) * half_dt;
+            if constexpr (S == 4) c[ch] = yv + (piv + ldo(K(b.k2, ch + 4), qb) * half_dt) * dt;
+        } else {
+            const T *kp = (S == 2) ? b.k1 : (S == 3) ? b.k2 : b.k3;
+            c[ch] = yv + ldo(K(kp, ch - 16), qb) * (S == 4 ? dt : half_dt);
+        }
+    }
+
+    const R gw = R(P.gw), gy = R(P.gy), gp2 = R(P.gp2), h = R(0.5) * gw, hy = R(0.5) * gy;
+    const R bw = R(P.bw), byc = R(P.by);
+    const R p[4] = {c[0], c[1], c[2], c[3]};
+    auto W = [&](int a, int i) -> R { return c[CH_W + 3 * a + i]; };
+    auto E = [&](int a, int i) -> R { return c[CH_E + 3 * a + i]; };
+    const R r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+    const R phipi = c[CH_PI] * p[0] + c[CH_PI + 1] * p[1] + c[CH_PI + 2] * p[2] + c[CH_PI + 3] * p[3];
+
+    // ---- accumulators
+    R ktphi[4], ktW[3][3], ktY[3], kG[3], kS, u[3] = {R(0), R(0), R(0)};
+    R Bw[3][3], BYv[3] = {R(0), R(0), R(0)};
+    {
+        const R pot = R(2) * R(P.lam) * (r2 - R(P.vev) * R(P.vev)) + R(0.5) * R(P.jex);
+        R Gp[4];
+        apply_gen(h, c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2], hy, c[CH_SIG], p, Gp);
+        const R inv_r2 = R(1) / r2;   // keeps the reference's 0/0 at |phi| = 0
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++)
+            ktphi[bb] = Gp[bb] - pot * p[bb] - R(P.gam) * p[bb] * (p[bb] * c[CH_PI + bb]) * inv_r2;
+        const R pi[4] = {c[CH_PI], c[CH_PI + 1], c[CH_PI + 2], c[CH_PI + 3]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            R na = R(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++) na -= xprod(a, W(0, i), W(1, i), W(2, i), E(0, i), E(1, i), E(2, i));
+            kG[a] = gp2 * gw * (na + phiT(a, p, pi));
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                ktW[a][i] = -gw * xprod(a, W(0, i), W(1, i), W(2, i), c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2]);
+        }
+        kS = gp2 * gy * phiTY(p, pi);
+#pragma unroll
+        for (int i = 0; i < 3; i++) ktY[i] = R(0);
+    }
+
+    // ---- staging of one channel into LDS: centre from registers, faces from
+    // global.  Per face class (0 = star, 1/2/3 = x/y/z faces) each thread owns
+    // up to MAXF halo elements; their in-channel byte offsets and LDS offsets
+    // are computed once.  An empty slot loads element 0 and writes it to the
+    // lane's trash slot past B::BOX, so the pipeline has no branches.
+    constexpr int MAXF = (face_count<TX, TY, TZ>(0) + B::NT - 1) / B::NT;
+    const R inv60dx = R(g.inv60dx), invdx2 = R(g.invdx2);
+    int goff[4][MAXF], loff[4][MAXF];
+#pragma unroll
+    for (int dir = 0; dir < 4; dir++) {
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            const int f = t + s * B::NT;
+            goff[dir][s] = 0;
+            loff[dir][s] = B::BOX + (t & 63);
+            if (f < face_count<TX, TY, TZ>(dir)) {
+                int lx, ly, lz;
+                face_elem<TX, TY, TZ>(dir, f, lx, ly, lz);
+                goff[dir][s] = (int)((wrapc(x0 + lx - 3, g.nx) + nxl * wrapc(y0 + ly - 3, g.ny) +
+                                      nxy * wrapc(z0 + lz - 3, g.nz)) * (long)sizeof(T));
+                loff[dir][s] = lz * B::HX * B::HY + ly * B::HX + lx;
+            }
+        }
+    }
+    const int lcen = (tz + 3) * B::HX * B::HY + (ty + 3) * B::HX + (tx + 3);
+
+    using Regs = PassRegs<T, MAXF>;
+    auto fetch = [&](auto pc, Regs &rg) {
+        constexpr int pp = decltype(pc)::value;
+        constexpr int ch = pass_channel(pp);
+        constexpr int dir = pass_dir(pp);
+        constexpr int nf = face_count<TX, TY, TZ>(dir);
+        const T *y0p = Y(ch);
+        const T *y1p = nullptr, *k2p = nullptr;
+        if constexpr (S > 1) {
+            if constexpr (ch < 16) {
+                y1p = Y(ch + 20);
+                if constexpr (S == 3) k2p = K(b.k1, ch + 4);
+                if constexpr (S == 4) k2p = K(b.k2, ch + 4);
+            } else {
+                const T *kp = (S == 2) ? b.k1 : (S == 3) ? b.k2 : b.k3;
+                y1p = K(kp, ch - 16);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            if (s * B::NT < nf) {
+                const unsigned o = (unsigned)goff[dir][s];
+                rg.v[s][0] = ldo(y0p, o);
+                if constexpr (S > 1) rg.v[s][1] = ldo(y1p, o);
+                if constexpr (S > 2 && ch < 16) rg.v[s][2] = ldo(k2p, o);
+            }
+        }
+    };
+    auto stage_value = [&](auto pc, const T (&v)[3]) -> R {
+        constexpr int ch = pass_channel(decltype(pc)::value);
+        if constexpr (S == 1) return v[0];
+        else if constexpr (ch < 16) {
+            if constexpr (S == 2) return v[0] + v[1] * half_dt;
+            else if constexpr (S == 3) return v[0] + (v[1] + v[2] * half_dt) * half_dt;
+            else return v[0] + (v[1] + v[2] * half_dt) * dt;
+        } else {
+            return v[0] + v[1] * (S == 4 ? dt : half_dt);
+        }
+    };
+    auto store = [&](auto pc, const Regs &rg, R *buf) {
+        constexpr int pp = decltype(pc)::value;
+        constexpr int dir = pass_dir(pp);
+        constexpr int nf = face_count<TX, TY, TZ>(dir);
+        buf[lcen] = c[pass_channel(pp)];
+#pragma unroll
+        for (int s = 0; s < MAXF; s++)
+            if (s * B::NT < nf) buf[loff[dir][s]] = stage_value(pc, rg.v[s]);
+    };
+
+    // derivatives of the staged channel at this thread's site
+    auto deriv = [&](const R *buf, int ax, R cen, bool second, R &d1, R &d2) {
+        const int stride = ax == 0 ? 1 : ax == 1 ? B::HX : B::HX * B::HY;
+        const R *o = buf + lcen;
+        const R m3 = o[-3 * stride], m2 = o[-2 * stride], m1 = o[-stride];
+        const R a1 = o[stride], a2 = o[2 * stride], a3 = o[3 * stride];
+        d1 = sten_d1(m3, m2, m1, a1, a2, a3, inv60dx);
+        if (second) d2 = sten_d2(m3, m2, m1, cen, a1, a2, a3, invdx2);
+    };
+
+    // fold the derivatives of pass pp into the accumulators
+    auto consume = [&](auto pc, const R *buf) {
+        constexpr int pp = decltype(pc)::value;
+        constexpr int kind = pass_kind(pp);
+        constexpr int ch = pass_channel(pp);
+        const R cen = c[ch];
+        if constexpr (kind == 0) {
+            // E^a_i, E^Y_i: d_i only, divergences into k_Gamma, k_Sigma (603-657)
+            constexpr int i = pp / 4, a = pp % 4;
+            R d1, d2;
+            deriv(buf, i, cen, false, d1, d2);
+            if constexpr (a < 3) kG[a] += (R(1) - gp2) * d1;
+            else kS += (R(1) - gp2) * d1;
+        } else if constexpr (kind == 1) {
+            // Gamma^a, Sigma: -d_i Gamma^a into kt_W^a_i, -d_i Sigma into kt_Y_i
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                R d1, d2;
+                deriv(buf, i, cen, false, d1, d2);
+                if constexpr (pp < 15) ktW[pp - 12][i] -= d1;
+                else ktY[i] -= d1;
+            }
+        } else if constexpr (kind == 2) {
+            // phi_b: kt_phi (Laplacian, A_i(d_i phi + D_i phi)), phi.T D into kt_W, kt_Y, u_j
+            constexpr int bb = pp - 16;
+            R lap = R(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                R d1, d2;
+                deriv(buf, i, cen, true, d1, d2);
+                lap += d2;
+                R Ap[4];
+                apply_gen(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], p, Ap);
+                const R D = d1 + Ap[bb];
+                add_gen_col<bb>(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], d1 + D, ktphi);
+                ktW[0][i] += gw * phiT_coef<0, bb>(p) * D;
+                ktW[1][i] += gw * phiT_coef<1, bb>(p) * D;
+                ktW[2][i] += gw * phiT_coef<2, bb>(p) * D;
+                ktY[i] += gy * phiTY_coef<bb>(p) * D;
+                if constexpr (BETA) u[i] += p[bb] * d1;
+            }
+            ktphi[bb] += lap;
+        } else if constexpr (kind == 3) {
+            // Y_i: Laplacian into kt_Y, d_j Y_i into B^Y (beta terms only)
+            constexpr int i = pp - 20;
+            R lap = R(0);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                R d1, d2;
+                deriv(buf, j, cen, true, d1, d2);
+                lap += d2;
+                if constexpr (BETA) {
+                    if (j != i) {
+                        if (i == (j + 1) % 3) BYv[(j + 2) % 3] += d1;
+                        else BYv[(j + 1) % 3] -= d1;
+                    }
+                }
+            }
+            ktY[i] += lap;
+            if constexpr (BETA && pp == 22) {
+                // B^Y complete: beta_Y terms (main_rk4.jl:354, 553-581)
+                const R QY = c[CH_EY] * BYv[0] + c[CH_EY + 1] * BYv[1] + c[CH_EY + 2] * BYv[2];
+#pragma unroll
+                for (int bq = 0; bq < 4; bq++) ktphi[bq] += byc * QY * p[bq];
+#pragma unroll
+                for (int ii = 0; ii < 3; ii++) {
+                    const int j = (ii + 1) % 3, k = (ii + 2) % 3;
+                    ktY[ii] += R(2) * byc * (u[j] * c[CH_EY + k] - u[k] * c[CH_EY + j] - phipi * BYv[ii]);
+                }
+            }
+        } else {
+            // W^a_i: Laplacian, -gw eps W^c_j d_j W^a_i into kt_W^{a'}_i, d_j W^a_i into B^a
+            constexpr int i = (pp - 23) / 3, a = (pp - 23) % 3;
+            if constexpr (pp == 23) {
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++)
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++) {
+                        const int j = (ii + 1) % 3, k = (ii + 2) % 3;
+                        Bw[aa][ii] = gw * xprod(aa, W(0, j), W(1, j), W(2, j), W(0, k), W(1, k), W(2, k));
+                    }
+            }
+            R lap = R(0);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                R d1, d2;
+                deriv(buf, j, cen, true, d1, d2);
+                lap += d2;
+                // (d_j W_i x W_j)^{a'} = eps^{a' a c} d_j W^a_i W^c_j
+                constexpr int a1 = (a + 1) % 3, a2 = (a + 2) % 3;   // a' values
+                ktW[a1][i] -= gw * R(eps3(a1, a)) * W(3 - a1 - a, j) * d1;
+                ktW[a2][i] -= gw * R(eps3(a2, a)) * W(3 - a2 - a, j) * d1;
+                if (j != i) {
+                    if (i == (j + 1) % 3) Bw[a][(j + 2) % 3] += d1;
+                    else Bw[a][(j + 1) % 3] -= d1;
+                }
+            }
+            ktW[a][i] += lap;
+            if constexpr (pp == 31) {
+                // B^a complete: -gw sum_j (W_j x F_ij)^a, beta_W terms
+                if constexpr (BETA) {
+                    R QW = R(0);
+#pragma unroll
+                    for (int aa = 0; aa < 3; aa++)
+                        QW += E(aa, 0) * Bw[aa][0] + E(0, 1) * Bw[aa][1] + E(0, 2) * Bw[aa][2];
+#pragma unroll
+                    for (int bq = 0; bq < 4; bq++) ktphi[bq] += bw * QW * p[bq];
+                }
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) {
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++) {
+                        const int j1 = (ii + 1) % 3, j2 = (ii + 2) % 3;
+                        // F_{i,i+1} = B_{i+2}, F_{i,i+2} = -B_{i+1}
+                        const R t2 = xprod(aa, W(0, j1), W(1, j1), W(2, j1), Bw[0][j2], Bw[1][j2], Bw[2][j2]) -
+                                     xprod(aa, W(0, j2), W(1, j2), W(2, j2), Bw[0][j1], Bw[1][j1], Bw[2][j1]);
+                        ktW[aa][ii] += -gw * t2;
+                        if constexpr (BETA)
+                            ktW[aa][ii] += R(2) * bw * (u[j1] * E(aa, j2) - u[j2] * E(aa, j1) - phipi * Bw[aa][ii]);
+                    }
+                }
+            }
+        }
+    };
+
+    // pin the accumulators still live after pass pp
+    auto pin_all = [&](auto pc) {
+        constexpr int pp = decltype(pc)::value;
+#pragma unroll
+        for (int i = 0; i < 4; i++) pin(ktphi[i]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            pin(ktY[i]); pin(kG[i]);
+            if constexpr (BETA && pp >= 16) pin(u[i]);
+            if constexpr (BETA && pp >= 20 && pp < 22) pin(BYv[i]);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                pin(ktW[i][j]);
+                if constexpr (pp >= 23 && pp < 31) pin(Bw[i][j]);
+            }
+        }
+        pin(kS);
+    };
+
+    // ---- the channel pipeline: the halos of passes p+1 .. p+PFS are in
+    // flight (in a ring of PFS register sets) while pass p is computed from
+    // LDS.  Set k % PFS holds pass k; the set of pass p is refilled with pass
+    // p + PFS once pass p has been staged (during pass p - 1).
+    constexpr int PFS = pf_of_stage(PF, S);
+    Regs rr[PFS];
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        (fetch(std::integral_constant<int, I>{}, rr[I]), ...);
+    }(std::make_integer_sequence<int, PFS>{});
+    store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
+    lds_barrier();
+    auto body = [&](auto pc) {
+        constexpr int pp = decltype(pc)::value;
+        if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
+        consume(pc, lds[pp & 1]);
+        pin_all(pc);
+        if constexpr (pp + 1 < NPASS) store(std::integral_constant<int, pp + 1>{}, rr[(pp + 1) % PFS], lds[(pp + 1) & 1]);
+        lds_barrier();
+    };
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        (body(std::integral_constant<int, I>{}), ...);
+    }(std::make_integer_sequence<int, NPASS>{});
+
+    // ---- outputs: k_Gamma(3), k_Sigma, kt_phi(4), kt_W(9), kt_Y(3)
+    R o[NK];
+#pragma unroll
+    for (int a = 0; a < 3; a++) o[a] = kG[a];
+    o[3] = kS;
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) o[4 + bb] = ktphi[bb];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) o[8 + 3 * a + i] = ktW[a][i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) o[17 + i] = ktY[i];
+
+    if constexpr (S < 4) {
+#pragma unroll
+        for (int m = 0; m < NK; m++) sto(b.kout + (long)m * b.n3, qb, o[m]);
+    } else {
+#pragma unroll
+        for (int m = 0; m < NK; m++) sto(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
+    }
+}
+
+template <int TX, int TY, int TZ>
+bool brick_fits(const ewpt_lattice &L)
+{
+    // byte offsets within one channel are 32-bit
+    return L.nx % TX == 0 && L.ny % TY == 0 && L.nz % TZ == 0 && L.nx >= TX && L.ny >= TY && L.nz >= TZ &&
+           (long)L.nx * L.ny * L.nz * 8 < (1L << 31);
+}
+
+template <int TX, int TY, int TZ, int PF, bool BETA>
+hipError_t launch_tile_beta(int stage, const StageBufs<double> &b, const BrickGeo &g, const Phys &P, double dt,
+                            hipStream_t st)
+{
+    using B = BrickShape<TX, TY, TZ>;
+    const dim3 grid(g.nbx * g.nby * g.nbz), block(B::NT);
+    const size_t lds13 = 2 * B::BUF * sizeof(double);
+    const size_t lds4 = lds13 + NK * B::NT * sizeof(double);
+    static bool attr_set = false;   // the stage-4 kernels need more than 64 KB of LDS
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    switch (stage) {
+    case 1: stage_brick<TX, TY, TZ, PF, BETA, 1><<<grid, block, lds13, st>>>(b, g, P, dt); break;
+    case 2: stage_brick<TX, TY, TZ, PF, BETA, 2><<<grid, block, lds13, st>>>(b, g, P, dt); break;
+    case 3: stage_brick<TX, TY, TZ, PF, BETA, 3><<<grid, block, lds13, st>>>(b, g, P, dt); break;
+    default: stage_brick<TX, TY, TZ, PF, BETA, 4><<<grid, block, lds4, st>>>(b, g, P, dt); break;
+    }
+    return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, int PF>
+hipError_t launch_tile(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                       int xcd_map, int order, hipStream_t st)
+{
+    BrickGeo g;
+    g.order = order;
+    g.nx = L.nx; g.ny = L.ny; g.nz = L.nz;
+    g.nbx = L.nx / TX; g.nby = L.ny / TY; g.nbz = L.nz / TZ;
+    const int nb = g.nbx * g.nby * g.nbz;
+    g.xcd_map = (xcd_map && nb % 8 == 0) ? 1 : 0;
+    g.inv60dx = 1.0 / (60.0 * L.dx);
+    g.invdx2 = 1.0 / (L.dx * L.dx);
+    if (P.bw != 0.0 || P.by != 0.0) return launch_tile_beta<TX, TY, TZ, PF, true>(stage, b, g, P, dt, st);
+    return launch_tile_beta<TX, TY, TZ, PF, false>(stage, b, g, P, dt, st);
+}
+
+}  // namespace
+
+}  // namespace ewpt

@@ -23,6 +23,7 @@ hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, d
 
 // ewpt_brick.hip -- LDS-tiled fused stage kernels, variants 1..5
 int brick_variant_count();
+bool brick_exists(int variant);
 bool brick_supported(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
                               const Phys &P, double dt, hipStream_t st);

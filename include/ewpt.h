@@ -118,10 +118,16 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 
 /* Kernel variant of the stage launches.  All variants compute the same
  * result; they exist for the autotuner and for measurements.
- *   0 = automatic (the 16x8x4 LDS brick when the lattice tiles by it, else 9)
- *   1..5 = LDS-brick stage kernel: 1 = 16x8x4, 2 = 8x8x8, 3 = 32x4x4,
- *          4 = 16x16x2 (XCD-aware block order), 5 = 16x8x4 (linear order)
+ *   0 = automatic: the first of 211, 210, 2 that tiles the lattice, else 9
+ *   1..8, 10, 11 = LDS-brick stage kernel (brick, prefetch, block order):
+ *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
+ *          5 = 16x8x4 pf1 linear order, 6 = 16x4x4 pf1, 7 = 8x8x4 pf2,
+ *          8 = 16x4x4 pf2, 10 = 16x8x4 and 11 = 32x4x4 with prefetch
+ *          depth 4/4/3/2 for stages 1..4; all but 5 use the XCD-aware
+ *          block order
  *   9 = gather kernel (one site per thread, any lattice size)
+ * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
+ * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows), e.g. 111.
  * Setting a brick variant that does not tile the lattice returns
  * EWPT_ERR_UNSUPPORTED. */
 int ewpt_set_variant(ewpt_ctx *ctx, int variant);

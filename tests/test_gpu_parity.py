@@ -109,11 +109,12 @@ def test_init_modes_matches_cpu_rebuild():
     assert np.all(np.abs(table[:, :, :3]) <= 2)
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 9])
-def test_stage_variants_match_oracle(variant):
+@pytest.mark.parametrize("beta", [0.0, 0.2])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 111, 211])
+def test_stage_variants_match_oracle(variant, beta):
     shape = (32, 32, 32)
-    print(f"variant {variant} shape={shape}", flush=True)
-    p = fields.params(beta_w=0.2, beta_y=0.1, gamma=0.3, jex=0.5)
+    print(f"variant {variant} beta {beta} shape={shape}", flush=True)
+    p = fields.params(beta_w=beta, beta_y=beta / 2, gamma=0.3, jex=0.5)
     s = fields.smooth_state(shape, seed=12, amp=0.5)
     with ewpt.Lattice(shape, DX) as lat:
         lat.variant = variant
