@@ -28,9 +28,17 @@ for _p in (os.path.join(ROOT, "ewpt-gpu_amd"), ROOT):
 
 METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline"
 HBM_PEAK = 8.0e12
-# algorithmic HBM bytes per site per stage launch, FP64 (stage-separated RK4
-# with 20 stored derivative channels per stage): reads+writes in channels
-FP64_STAGE_CH = {1: 36 + 20, 2: 36 + 20 + 20, 3: 36 + 16 + 20 + 20, 4: 36 + 60 + 36}
+# Algorithmic HBM bytes per site per stage launch (stage-separated RK4 with 20
+# stored derivative channels per stage), as channel accesses times the word
+# size.  FP64: S1 reads y, writes k1; S2 reads y, k1, writes k2; S3 reads y,
+# the 16 momentum k1's, k2, writes k3; S4 reads y, k1..k3, writes y'.  Mixed:
+# the same in FP32 plus the FP32 compensation word of y, read and written at
+# stage 4 only.
+STAGE_BYTES = {
+    "fp64": {1: (36 + 20) * 8, 2: (36 + 20 + 20) * 8, 3: (36 + 16 + 20 + 20) * 8, 4: (36 + 60 + 36) * 8},
+    "mixed": {1: (36 + 20) * 4, 2: (36 + 20 + 20) * 4, 3: (36 + 16 + 20 + 20) * 4,
+              4: (36 + 36 + 60 + 36 + 36) * 4},
+}
 
 
 def cpu_baseline(n: int = 64, steps: int = 12):
@@ -108,8 +116,7 @@ def main(argv=None) -> int:
     e = lat.energies(P)
     lat.close()
 
-    bytes_per = 8 if prec == _lib.EWPT_FP64 else 4
-    stage_bytes = {s: FP64_STAGE_CH[s] * bytes_per * sites for s in (1, 2, 3, 4)}
+    stage_bytes = {s: STAGE_BYTES[a.precision][s] * sites for s in (1, 2, 3, 4)}
     dom = max(range(4), key=lambda s: stage_ms[s]) + 1
     achieved = stage_bytes[dom] / (stage_ms[dom - 1] * 1e-3)
     step_bytes = sum(stage_bytes.values())

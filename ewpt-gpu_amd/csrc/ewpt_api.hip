@@ -23,8 +23,13 @@ struct ewpt_ctx {
     ewpt_lattice lat{};
     long n3 = 0;
     hipStream_t stream = nullptr;
-    // FP64 path: state ping-pong and the three stored stage derivatives
+    // FP64 path: state ping-pong and the three stored stage derivatives.
+    // Mixed path: y is an FP64 view of the state (for state I/O and the
+    // FP64 reductions), rebuilt from the FP32 pair when view_valid is false.
     double *y = nullptr, *ynext = nullptr, *k[3] = {nullptr, nullptr, nullptr};
+    float *fhi = nullptr, *flo = nullptr, *fhi_n = nullptr, *flo_n = nullptr;
+    float *fk[3] = {nullptr, nullptr, nullptr};
+    bool view_valid = true;
     // host-buffer RHS path (allocated on first use)
     double *rhs_in = nullptr, *rhs_out = nullptr;
     // reductions
@@ -58,15 +63,16 @@ int fail(ewpt_ctx *c, int code, const char *what, hipError_t e = hipSuccess)
         if (e_ != hipSuccess) return fail(ctx, EWPT_ERR_HIP, what, e_); \
     } while (0)
 
-int dev_alloc(ewpt_ctx *c, double **p, size_t ndouble, const char *what)
+template <typename T>
+int dev_alloc(ewpt_ctx *c, T **p, size_t n, const char *what)
 {
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), ndouble * sizeof(double));
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return fail(c, e == hipErrorOutOfMemory ? EWPT_ERR_OOM : EWPT_ERR_HIP, what, e);
     }
-    c->bytes += ndouble * sizeof(double);
+    c->bytes += n * sizeof(T);
     return EWPT_OK;
 }
 
@@ -74,6 +80,9 @@ void release(ewpt_ctx *c)
 {
     double **bufs[] = {&c->y, &c->ynext, &c->k[0], &c->k[1], &c->k[2], &c->rhs_in, &c->rhs_out, &c->part, &c->res};
     for (double **b : bufs)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
+    float **fbufs[] = {&c->fhi, &c->flo, &c->fhi_n, &c->flo_n, &c->fk[0], &c->fk[1], &c->fk[2]};
+    for (float **b : fbufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (c->stream) { (void)hipStreamDestroy(c->stream); c->stream = nullptr; }
 }
@@ -108,12 +117,37 @@ StageBufs<double> bufs_for(ewpt_ctx *c, int stage)
 
 int set_device(ewpt_ctx *c) { HIPCHK(c, hipSetDevice(c->device), "hipSetDevice"); return EWPT_OK; }
 
+bool mixed(const ewpt_ctx *c) { return c->precision == EWPT_MIXED; }
+
+// mixed path: refresh the FP64 view from the FP32 pair
+int ensure_view(ewpt_ctx *c)
+{
+    if (!mixed(c) || c->view_valid) return EWPT_OK;
+    HIPCHK(c, launch_join(c->fhi, c->flo, c->y, NCH * c->n3, c->stream), "join state");
+    c->view_valid = true;
+    return EWPT_OK;
+}
+
+// mixed path: the FP64 view was written; split it into the FP32 pair
+int push_view(ewpt_ctx *c)
+{
+    if (!mixed(c)) return EWPT_OK;
+    HIPCHK(c, launch_split(c->y, c->fhi, c->flo, NCH * c->n3, c->stream), "split state");
+    c->view_valid = true;
+    return EWPT_OK;
+}
+
 // variant actually launched: AUTO takes the first brick variant that tiles
 // the lattice from the measured ranking (512^3 FP64 on MI355X: 211, 210, 2),
 // else the gather kernel
 int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
+    if (mixed(c)) {
+        for (int v : {211, 210})
+            if (brick_supported_f32(v, c->lat)) return v;
+        return -1;
+    }
     for (int v : {211, 210, 2})
         if (brick_supported(v, c->lat)) return v;
     return VARIANT_GATHER;
@@ -123,6 +157,24 @@ int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
 {
     const Phys P = phys_of(*p);
     const int v = resolved_variant(c);
+    if (mixed(c)) {
+        StageBufs<float> b{};
+        b.y = c->fhi;
+        b.ylo = c->flo;
+        b.k1 = c->fk[0];
+        b.k2 = c->fk[1];
+        b.k3 = c->fk[2];
+        b.n3 = c->n3;
+        if (stage < 4) b.kout = c->fk[stage - 1];
+        else { b.ynew = c->fhi_n; b.ylo_new = c->flo_n; }
+        HIPCHK(c, launch_stage_brick_f32(v, stage, b, c->lat, P, p->dt, c->stream), "stage launch");
+        if (stage == 4) {
+            std::swap(c->fhi, c->fhi_n);
+            std::swap(c->flo, c->flo_n);
+            c->view_valid = false;
+        }
+        return EWPT_OK;
+    }
     if (v == VARIANT_GATHER)
         HIPCHK(c, launch_stage_gather(stage, bufs_for(c, stage), c->lat, P, p->dt, c->stream), "stage launch");
     else
@@ -159,7 +211,8 @@ int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devi
     *out = nullptr;
     if (!lattice_ok(lat) || (precision != EWPT_FP64 && precision != EWPT_MIXED) || device < 0)
         return EWPT_ERR_INVAL;
-    if (precision == EWPT_MIXED) return EWPT_ERR_UNSUPPORTED;
+    if (precision == EWPT_MIXED && !brick_supported_f32(210, *lat) && !brick_supported_f32(211, *lat))
+        return EWPT_ERR_UNSUPPORTED;   // the mixed path has LDS-brick kernels only
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev) {
         (void)hipGetLastError();
@@ -176,8 +229,15 @@ int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devi
         rc = fail(c, EWPT_ERR_HIP, "hipStreamCreate");
     const size_t n3 = (size_t)c->n3;
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->y, NCH * n3, "alloc y");
-    if (rc == EWPT_OK) rc = dev_alloc(c, &c->ynext, NCH * n3, "alloc y'");
-    for (int s = 0; s < 3 && rc == EWPT_OK; s++) rc = dev_alloc(c, &c->k[s], NK * n3, "alloc k");
+    if (precision == EWPT_FP64) {
+        if (rc == EWPT_OK) rc = dev_alloc(c, &c->ynext, NCH * n3, "alloc y'");
+        for (int s = 0; s < 3 && rc == EWPT_OK; s++) rc = dev_alloc(c, &c->k[s], NK * n3, "alloc k");
+    } else {
+        float **st[] = {&c->fhi, &c->flo, &c->fhi_n, &c->flo_n};
+        for (float **b : st)
+            if (rc == EWPT_OK) rc = dev_alloc(c, b, NCH * n3, "alloc y (fp32)");
+        for (int s = 0; s < 3 && rc == EWPT_OK; s++) rc = dev_alloc(c, &c->fk[s], NK * n3, "alloc k (fp32)");
+    }
     c->nred = reduce_blocks(c->lat);
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->part, (size_t)c->nred * NRED_SLOTS, "alloc partials");
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->res, NRED_SLOTS, "alloc results");
@@ -207,6 +267,7 @@ int ewpt_set_state(ewpt_ctx *ctx, const double *host)
     if (rc) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->y, host, NCH * ctx->n3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
            "set_state copy");
+    if ((rc = push_view(ctx))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "set_state sync");
     ctx->has_state = true;
     return EWPT_OK;
@@ -218,6 +279,7 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host)
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "get_state before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(host, ctx->y, NCH * ctx->n3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
            "get_state copy");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "get_state sync");
@@ -239,6 +301,7 @@ int ewpt_init_modes(ewpt_ctx *ctx, unsigned long long seed, int nmodes, int kmax
     HIPCHK(ctx, hipMalloc(reinterpret_cast<void **>(&dt), nt * sizeof(double)), "alloc mode table");
     hipError_t e = hipMemcpyAsync(dt, t.data(), nt * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_fill_modes(ctx->y, ctx->lat, dt, nmodes, vev, ctx->stream);
+    if (e == hipSuccess && mixed(ctx)) e = launch_split(ctx->y, ctx->fhi, ctx->flo, NCH * ctx->n3, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(dt);
     if (e != hipSuccess) return fail(ctx, EWPT_ERR_HIP, "init_modes", e);
@@ -326,6 +389,7 @@ int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13])
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "energies before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
     HIPCHK(ctx, launch_energies(ctx->y, ctx->lat, phys_of(*p), ctx->part, ctx->res, ctx->stream), "energy launch");
     double r[16];
     HIPCHK(ctx, hipMemcpyAsync(r, ctx->res, 10 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "energy copy");
@@ -348,6 +412,7 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8])
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "gauss before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
     HIPCHK(ctx, launch_gauss(ctx->y, ctx->lat, phys_of(*p), ctx->part, ctx->res, ctx->stream), "gauss launch");
     double r[8];
     HIPCHK(ctx, hipMemcpyAsync(r, ctx->res, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "gauss copy");
@@ -365,7 +430,19 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8])
 int ewpt_set_variant(ewpt_ctx *ctx, int variant)
 {
     if (!ctx) return EWPT_ERR_INVAL;
-    if (variant == VARIANT_AUTO || variant == VARIANT_GATHER) {
+    if (variant == VARIANT_AUTO) {
+        ctx->variant = variant;
+        return EWPT_OK;
+    }
+    if (mixed(ctx)) {
+        if (variant % 100 != 10 && variant % 100 != 11) return EWPT_ERR_INVAL;
+        if (!brick_exists(variant)) return EWPT_ERR_INVAL;
+        if (!brick_supported_f32(variant, ctx->lat))
+            return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
+        ctx->variant = variant;
+        return EWPT_OK;
+    }
+    if (variant == VARIANT_GATHER) {
         ctx->variant = variant;
         return EWPT_OK;
     }

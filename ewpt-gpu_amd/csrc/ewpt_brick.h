@@ -196,6 +196,15 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void pin(float &x) { asm volatile("" : "+v"(x)); }
 
+// Knuth's TwoSum: s + e == a + b exactly (round-to-nearest, no reassociation)
+template <typename R>
+__device__ __forceinline__ void two_sum(R a, R b, R &s, R &e)
+{
+    s = a + b;
+    const R bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
 // sign of eps^{a b c} for the third index c
 __host__ __device__ constexpr int eps3(int a, int b) { return ((b - a + 3) % 3 == 1) ? 1 : -1; }
 
@@ -213,13 +222,21 @@ __host__ __device__ constexpr int pf_of_stage(int PF, int S)
     return PF < 10 ? PF : (PF / (S == 1 ? 1000 : S == 2 ? 100 : S == 3 ? 10 : 1)) % 10;
 }
 
-// T: storage and arithmetic type (double here).  PF: prefetch profile
-// (pf_of_stage).  Two waves per SIMD are requested so that the compiler
-// keeps within 256 VGPRs also for 256-thread bricks (two workgroups per CU).
+// T: storage and arithmetic type (double, or float for the mixed path).
+// PF: prefetch profile (pf_of_stage).  Two waves per SIMD are requested for
+// FP64 (256 VGPRs, also for 256-thread bricks at two workgroups per CU) and
+// four for FP32 stages 1-3 (two 512-thread workgroups per CU); the FP32
+// stage 4 parks more in LDS and runs one workgroup per CU.
 template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T = double>
-__global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(TX *TY *TZ)
+__attribute__((amdgpu_waves_per_eu(std::is_same_v<T, float> && S < 4 ? 4 : 2)))
 stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 {
+    // T = float is the mixed-precision path: y is stored as an FP32 pair
+    // (b.y = high word, b.ylo = compensation word), stages read the high
+    // word only, and stage 4 adds the step increment with a compensated sum:
+    // t = lo + dy, then (hi', lo') = TwoSum(hi, t) (Knuth's error-free sum).
+    constexpr bool MIXED = std::is_same_v<T, float>;
     using B = BrickShape<TX, TY, TZ>;
     using R = T;
     // dynamic LDS: two staging buffers, then (stage 4) the 20 partial sums n
@@ -254,9 +271,18 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
     if constexpr (S == 4) {
 #pragma unroll
         for (int m = 0; m < NK; m++) {
-            R n = ldo(K(b.k1, m), qb) * d6 + ldo(Y(16 + m), qb);
-            n = ldo(K(b.k2, m), qb) * d3 + n;
-            nstash[m * B::NT + t] = ldo(K(b.k3, m), qb) * d3 + n;
+            if constexpr (MIXED) {
+                // increment without k4, plus the compensation word
+                R dy = ldo(K(b.k1, m), qb) * d6;
+                dy = ldo(K(b.k2, m), qb) * d3 + dy;
+                dy = ldo(K(b.k3, m), qb) * d3 + dy;
+                nstash[m * B::NT + t] = ldo(b.ylo + (long)(16 + m) * b.n3, qb) + dy;
+                nstash[(NK + m) * B::NT + t] = ldo(Y(16 + m), qb);
+            } else {
+                R n = ldo(K(b.k1, m), qb) * d6 + ldo(Y(16 + m), qb);
+                n = ldo(K(b.k2, m), qb) * d3 + n;
+                nstash[m * B::NT + t] = ldo(K(b.k3, m), qb) * d3 + n;
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -265,10 +291,21 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
             const R k2 = piv + ldo(K(b.k1, ch + 4), qb) * half_dt;
             const R k3 = piv + ldo(K(b.k2, ch + 4), qb) * half_dt;
             const R k4 = piv + ldo(K(b.k3, ch + 4), qb) * dt;
-            R n = piv * d6 + ldo(Y(ch), qb);
-            n = k2 * d3 + n;
-            n = k3 * d3 + n;
-            sto(b.ynew + (long)ch * b.n3, qb, k4 * d6 + n);
+            if constexpr (MIXED) {
+                R dy = piv * d6;
+                dy = k2 * d3 + dy;
+                dy = k3 * d3 + dy;
+                dy = k4 * d6 + dy;
+                R nh, nl;
+                two_sum(ldo(Y(ch), qb), ldo(b.ylo + (long)ch * b.n3, qb) + dy, nh, nl);
+                sto(b.ynew + (long)ch * b.n3, qb, nh);
+                sto(b.ylo_new + (long)ch * b.n3, qb, nl);
+            } else {
+                R n = piv * d6 + ldo(Y(ch), qb);
+                n = k2 * d3 + n;
+                n = k3 * d3 + n;
+                sto(b.ynew + (long)ch * b.n3, qb, k4 * d6 + n);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -595,6 +632,14 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
     if constexpr (S < 4) {
 #pragma unroll
         for (int m = 0; m < NK; m++) sto(b.kout + (long)m * b.n3, qb, o[m]);
+    } else if constexpr (MIXED) {
+#pragma unroll
+        for (int m = 0; m < NK; m++) {
+            R nh, nl;
+            two_sum(nstash[(NK + m) * B::NT + t], o[m] * d6 + nstash[m * B::NT + t], nh, nl);
+            sto(b.ynew + (long)(16 + m) * b.n3, qb, nh);
+            sto(b.ylo_new + (long)(16 + m) * b.n3, qb, nl);
+        }
     } else {
 #pragma unroll
         for (int m = 0; m < NK; m++) sto(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
@@ -604,37 +649,37 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 template <int TX, int TY, int TZ>
 bool brick_fits(const ewpt_lattice &L)
 {
-    // byte offsets within one channel are 32-bit
+    // byte offsets within one channel are 32-bit (FP64 bound, also used for FP32)
     return L.nx % TX == 0 && L.ny % TY == 0 && L.nz % TZ == 0 && L.nx >= TX && L.ny >= TY && L.nz >= TZ &&
            (long)L.nx * L.ny * L.nz * 8 < (1L << 31);
 }
 
-template <int TX, int TY, int TZ, int PF, bool BETA>
-hipError_t launch_tile_beta(int stage, const StageBufs<double> &b, const BrickGeo &g, const Phys &P, double dt,
-                            hipStream_t st)
+template <int TX, int TY, int TZ, int PF, bool BETA, typename T>
+hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, hipStream_t st)
 {
     using B = BrickShape<TX, TY, TZ>;
     const dim3 grid(g.nbx * g.nby * g.nbz), block(B::NT);
-    const size_t lds13 = 2 * B::BUF * sizeof(double);
-    const size_t lds4 = lds13 + NK * B::NT * sizeof(double);
-    static bool attr_set = false;   // the stage-4 kernels need more than 64 KB of LDS
+    const size_t lds13 = 2 * B::BUF * sizeof(T);
+    // stage 4 parks n (FP64) or the increment and the high word (mixed)
+    const size_t lds4 = lds13 + (std::is_same_v<T, float> ? 2 : 1) * NK * B::NT * sizeof(T);
+    static bool attr_set = false;   // the stage-4 kernels may need more than 64 KB of LDS
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4, T>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     switch (stage) {
-    case 1: stage_brick<TX, TY, TZ, PF, BETA, 1><<<grid, block, lds13, st>>>(b, g, P, dt); break;
-    case 2: stage_brick<TX, TY, TZ, PF, BETA, 2><<<grid, block, lds13, st>>>(b, g, P, dt); break;
-    case 3: stage_brick<TX, TY, TZ, PF, BETA, 3><<<grid, block, lds13, st>>>(b, g, P, dt); break;
-    default: stage_brick<TX, TY, TZ, PF, BETA, 4><<<grid, block, lds4, st>>>(b, g, P, dt); break;
+    case 1: stage_brick<TX, TY, TZ, PF, BETA, 1, T><<<grid, block, lds13, st>>>(b, g, P, dt); break;
+    case 2: stage_brick<TX, TY, TZ, PF, BETA, 2, T><<<grid, block, lds13, st>>>(b, g, P, dt); break;
+    case 3: stage_brick<TX, TY, TZ, PF, BETA, 3, T><<<grid, block, lds13, st>>>(b, g, P, dt); break;
+    default: stage_brick<TX, TY, TZ, PF, BETA, 4, T><<<grid, block, lds4, st>>>(b, g, P, dt); break;
     }
     return hipGetLastError();
 }
 
-template <int TX, int TY, int TZ, int PF>
-hipError_t launch_tile(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+template <int TX, int TY, int TZ, int PF, typename T = double>
+hipError_t launch_tile(int stage, const StageBufs<T> &b, const ewpt_lattice &L, const Phys &P, double dt,
                        int xcd_map, int order, hipStream_t st)
 {
     BrickGeo g;
@@ -645,8 +690,8 @@ hipError_t launch_tile(int stage, const StageBufs<double> &b, const ewpt_lattice
     g.xcd_map = (xcd_map && nb % 8 == 0) ? 1 : 0;
     g.inv60dx = 1.0 / (60.0 * L.dx);
     g.invdx2 = 1.0 / (L.dx * L.dx);
-    if (P.bw != 0.0 || P.by != 0.0) return launch_tile_beta<TX, TY, TZ, PF, true>(stage, b, g, P, dt, st);
-    return launch_tile_beta<TX, TY, TZ, PF, false>(stage, b, g, P, dt, st);
+    if (P.bw != 0.0 || P.by != 0.0) return launch_tile_beta<TX, TY, TZ, PF, true, T>(stage, b, g, P, T(dt), st);
+    return launch_tile_beta<TX, TY, TZ, PF, false, T>(stage, b, g, P, T(dt), st);
 }
 
 }  // namespace

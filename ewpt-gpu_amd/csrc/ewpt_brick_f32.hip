@@ -1,0 +1,29 @@
+// ewpt_brick_f32.hip -- mixed-precision stage kernels: FP32 storage and
+// arithmetic with the compensated stage-4 update (ewpt_brick.h, T = float).
+//   10 (+100*order): 16x8x4 brick, prefetch depth 4/4/3/2
+//   11 (+100*order): 32x4x4 brick, prefetch depth 4/4/3/2
+#include "ewpt_brick.h"
+
+namespace ewpt {
+
+bool brick_supported_f32(int variant, const ewpt_lattice &L)
+{
+    switch (variant % 100) {
+    case 10: return brick_fits<16, 8, 4>(L);
+    case 11: return brick_fits<32, 4, 4>(L);
+    default: return false;
+    }
+}
+
+hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float> &b, const ewpt_lattice &L,
+                                  const Phys &P, double dt, hipStream_t st)
+{
+    const int order = variant / 100;
+    switch (variant % 100) {
+    case 10: return launch_tile<16, 8, 4, 4432, float>(stage, b, L, P, dt, 1, order, st);
+    case 11: return launch_tile<32, 4, 4, 4432, float>(stage, b, L, P, dt, 1, order, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ewpt

@@ -28,6 +28,15 @@ bool brick_supported(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
                               const Phys &P, double dt, hipStream_t st);
 
+// ewpt_brick_f32.hip -- mixed-precision stage kernels (variants 10, 11 + 100*order)
+bool brick_supported_f32(int variant, const ewpt_lattice &L);
+hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float> &b, const ewpt_lattice &L,
+                                  const Phys &P, double dt, hipStream_t st);
+
+// ewpt_mixed.hip
+hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream_t st);
+hipError_t launch_join(const float *hi, const float *lo, double *y, long n, hipStream_t st);
+
 // stage kernel variants selectable through ewpt_set_variant
 constexpr int VARIANT_AUTO = 0;
 constexpr int VARIANT_GATHER = 9;

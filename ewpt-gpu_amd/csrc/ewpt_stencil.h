@@ -28,8 +28,15 @@ __device__ __forceinline__ R sten_d1(R m3, R m2, R m1, R a1, R a2, R a3, R inv60
 template <typename R>
 __device__ __forceinline__ R sten_d2(R m3, R m2, R m1, R a0, R a1, R a2, R a3, R invdx2)
 {
-    return (R(1.5) * (a1 + m1) - R(0.15) * (a2 + m2) + R(1.0 / 90.0) * (a3 + m3)
-            - R(49.0 / 18.0) * a0) * invdx2;
+    if constexpr (sizeof(R) == 4) {
+        // FP32: taps relative to the centre (the weights sum to zero), so a
+        // smooth field loses no digits to the 49/18 a0 cancellation
+        return (R(1.5) * ((a1 - a0) + (m1 - a0)) - R(0.15) * ((a2 - a0) + (m2 - a0)) +
+                R(1.0 / 90.0) * ((a3 - a0) + (m3 - a0))) * invdx2;
+    } else {
+        return (R(1.5) * (a1 + m1) - R(0.15) * (a2 + m2) + R(1.0 / 90.0) * (a3 + m3)
+                - R(49.0 / 18.0) * a0) * invdx2;
+    }
 }
 
 __device__ __forceinline__ int wrap_idx(int i, int n)
