@@ -14,8 +14,9 @@ namespace {
 __global__ void __launch_bounds__(256) fill_modes(double *y, long n3, int nx, int ny, int nz, const double *modes,
                                                  int nmodes, double vev)
 {
-    const long q = blockIdx.x * 256L + threadIdx.x;
-    if (q >= n3) return;
+    // grid-stride: a lattice may hold more sites than one dispatch's 32-bit
+    // work-item count
+    for (long q = blockIdx.x * 256L + threadIdx.x; q < n3; q += (long)gridDim.x * 256L) {
     const int i = (int)(q % nx);
     const int j = (int)((q / nx) % ny);
     const int k = (int)(q / ((long)nx * ny));
@@ -27,6 +28,7 @@ __global__ void __launch_bounds__(256) fill_modes(double *y, long n3, int nx, in
             v += t[4] * cos(t[0] * tx + t[1] * ty + t[2] * tz + t[3]);
         }
         y[(long)c * n3 + q] = v;
+    }
     }
 }
 
@@ -58,7 +60,8 @@ hipError_t launch_fill_modes(double *y, const ewpt_lattice &L, const double *dmo
                              hipStream_t st)
 {
     const long n3 = (long)L.nx * L.ny * L.nz;
-    fill_modes<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(y, n3, L.nx, L.ny, L.nz, dmodes, nmodes, vev);
+    const long nb = (n3 + 255) / 256;
+    fill_modes<<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0, st>>>(y, n3, L.nx, L.ny, L.nz, dmodes, nmodes, vev);
     return hipGetLastError();
 }
 

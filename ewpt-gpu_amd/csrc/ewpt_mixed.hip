@@ -8,34 +8,43 @@ namespace ewpt {
 
 namespace {
 
+// A whole 36-channel state can exceed 2^32 elements (512^3: 4.8e9), more
+// work-items than one dispatch holds, so these kernels stride over it.
+constexpr unsigned CONV_BLOCKS = 16384;
+
 __global__ void __launch_bounds__(256) split_kernel(const double *y, float *hi, float *lo, long n)
 {
-    const long i = blockIdx.x * 256L + threadIdx.x;
-    if (i >= n) return;
-    const double v = y[i];
-    const float h = (float)v;
-    hi[i] = h;
-    lo[i] = (float)(v - (double)h);
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+        const double v = y[i];
+        const float h = (float)v;
+        hi[i] = h;
+        lo[i] = (float)(v - (double)h);
+    }
 }
 
 __global__ void __launch_bounds__(256) join_kernel(const float *hi, const float *lo, double *y, long n)
 {
-    const long i = blockIdx.x * 256L + threadIdx.x;
-    if (i >= n) return;
-    y[i] = (double)hi[i] + (double)lo[i];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L)
+        y[i] = (double)hi[i] + (double)lo[i];
+}
+
+unsigned conv_grid(long n)
+{
+    const long b = (n + 255) / 256;
+    return (unsigned)(b < (long)CONV_BLOCKS ? b : (long)CONV_BLOCKS);
 }
 
 }  // namespace
 
 hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream_t st)
 {
-    split_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(y, hi, lo, n);
+    split_kernel<<<conv_grid(n), 256, 0, st>>>(y, hi, lo, n);
     return hipGetLastError();
 }
 
 hipError_t launch_join(const float *hi, const float *lo, double *y, long n, hipStream_t st)
 {
-    join_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(hi, lo, y, n);
+    join_kernel<<<conv_grid(n), 256, 0, st>>>(hi, lo, y, n);
     return hipGetLastError();
 }
 

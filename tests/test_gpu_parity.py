@@ -185,3 +185,19 @@ def test_mixed_rejects_gather_variant():
     with ewpt.Lattice(32, DX, precision=ewpt.EWPT_MIXED) as lat:
         with pytest.raises(ewpt.EwptError):
             lat.variant = 9
+
+
+def test_mixed_tracks_fp64_at_512():
+    """Regression: at 512^3 a 36-channel state holds 4.8e9 values, more than
+    one dispatch's 32-bit work-item count; the FP64 <-> FP32-pair conversion
+    must cover all of it.  Mixed and FP64 energies after one step agree."""
+    n = 512
+    P = fields.params(gy=0.35)
+    e = {}
+    for prec in (ewpt.EWPT_FP64, ewpt.EWPT_MIXED):
+        print(f"512^3 precision {prec}", flush=True)
+        with ewpt.Lattice(n, DX, precision=prec) as lat:
+            lat.init_modes(seed=2024, nmodes=4, kmax=3, amp=0.3)
+            lat.step(P, 1)
+            e[prec] = lat.energies(P)
+    np.testing.assert_allclose(e[ewpt.EWPT_MIXED][:7], e[ewpt.EWPT_FP64][:7], rtol=1e-8)
