@@ -3,8 +3,9 @@
 
 One "step" is one full RK4 step (4 stage launches) of the 36-channel
 Higgs-gauge state on an N^3 periodic lattice (default 512^3, FP64), with the
-state resident in HBM before the timed region starts.  The state is a seeded
-synthetic smooth field generated on the device.
+state resident in HBM before the timed region starts.  The state is the
+seeded thermal initial condition (Bose-Einstein modes at T = 0.5, built on
+the device with hipFFT); ``--ic modes`` uses a smooth synthetic field instead.
 
 Prints ONE JSON line (rank 0):
   value      = site-updates/s (lattice sites x RK4 steps / second)
@@ -84,6 +85,8 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="both = FP64 headline value plus the mixed path in the same run")
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--ic", choices=("thermal", "modes"), default="thermal",
+                    help="initial state: thermal IC (T=0.5) or a smooth seeded mode sum")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--stream-copy", action="store_true",
                     help="only time a plain device copy (HBM ceiling / counter calibration)")
@@ -102,7 +105,9 @@ def main(argv=None) -> int:
 
     n = a.n
     dx = 0.25
-    P = ewpt.Parameters(N=n, dx=dx, gy=0.35).step_params()
+    T = 0.5
+    params = ewpt.Parameters(N=n, dx=dx, gy=0.35, T=T)
+    P = params.step_params()
     sites = n ** 3
 
     def run(precision: str, variant: int) -> dict:
@@ -112,7 +117,10 @@ def main(argv=None) -> int:
         lat = ewpt.Lattice(n, dx, precision=prec)
         try:
             lat.variant = variant
-            lat.init_modes(seed=2024, nmodes=4, kmax=3, amp=0.3)
+            if a.ic == "thermal":
+                lat.init_thermal(T, params.meff_sq, params.base_seed, "config")
+            else:
+                lat.init_modes(seed=2024, nmodes=4, kmax=3, amp=0.3)
             if a.warmup:
                 lat.step(P, a.warmup)
             stage_ms, total_ms = lat.step_timed(P, a.steps)   # synchronises before returning
@@ -149,10 +157,11 @@ def main(argv=None) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if precs[0] == "fp64" else "f32+f64",
-        "data": "synthetic (seeded smooth Fourier-mode fields generated on the device)",
+        "data": ("synthetic: seeded thermal IC (Bose-Einstein modes, T=0.5) built on the device"
+                 if a.ic == "thermal" else "synthetic: seeded smooth Fourier-mode fields generated on the device"),
         "config": {"workload": f"RK4 step of the 36-channel Higgs-gauge lattice, {n}^3 periodic, "
-                               f"{precs[0]}, dx=0.25, dt=dx/10",
-                   "lattice": n, "precision": precs[0], "variant": a.variant},
+                               f"{precs[0]}, dx=0.25, dt=dx/10, {a.ic} initial state",
+                   "lattice": n, "precision": precs[0], "variant": a.variant, "ic": a.ic},
         "roofline": head["roofline"],
         "step_roofline_frac": head["step_roofline_frac"],
         "stage_ms": head["stage_ms"],

@@ -310,6 +310,52 @@ int ewpt_init_modes(ewpt_ctx *ctx, unsigned long long seed, int nmodes, int kmax
     return EWPT_OK;
 }
 
+int ewpt_init_bubbles(ewpt_ctx *ctx, const double *bubbles, int nb, int profile, double mH)
+{
+    if (!ctx || nb < 0 || nb > (1 << 20) || (nb > 0 && !bubbles) || (profile != 0 && profile != 1) ||
+        !std::isfinite(mH) || !(mH > 0.0))
+        return EWPT_ERR_INVAL;
+    for (long q = 0; q < 7L * nb; q++)
+        if (!std::isfinite(bubbles[q])) return fail(ctx, EWPT_ERR_INVAL, "init_bubbles: non-finite bubble table");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    double *db = nullptr;
+    if (nb > 0) HIPCHK(ctx, hipMalloc(reinterpret_cast<void **>(&db), 7L * nb * sizeof(double)), "alloc bubbles");
+    hipError_t e = hipMemsetAsync(ctx->y, 0, NCH * ctx->n3 * sizeof(double), ctx->stream);
+    if (e == hipSuccess && nb > 0)
+        e = hipMemcpyAsync(db, bubbles, 7L * nb * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && nb > 0) e = launch_add_bubbles(ctx->y, ctx->lat, db, nb, profile, mH, ctx->stream);
+    if (e == hipSuccess && mixed(ctx)) e = launch_split(ctx->y, ctx->fhi, ctx->flo, NCH * ctx->n3, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (db) (void)hipFree(db);
+    if (e != hipSuccess) return fail(ctx, EWPT_ERR_HIP, "init_bubbles", e);
+    ctx->view_valid = true;
+    ctx->has_state = true;
+    return EWPT_OK;
+}
+
+int ewpt_init_thermal(ewpt_ctx *ctx, double T, double meff_sq, unsigned long long seed, int profile)
+{
+    if (!ctx || !std::isfinite(T) || !(T > 0.0) || !std::isfinite(meff_sq) || (profile != 0 && profile != 1))
+        return EWPT_ERR_INVAL;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    // the stored stage derivatives are free between steps: they hold the
+    // 3 N^3 complex transform buffers (FP64: 20 N^3 doubles, mixed: 20 N^3
+    // floats, both >= 6 N^3 doubles)
+    void *work = mixed(ctx) ? static_cast<void *>(ctx->fk[0]) : static_cast<void *>(ctx->k[0]);
+    HIPCHK(ctx, hipMemsetAsync(ctx->y, 0, NCH * ctx->n3 * sizeof(double), ctx->stream), "init_thermal clear");
+    const char *what = "";
+    if (thermal_state(ctx->y, ctx->lat, T, meff_sq, seed, profile, work, ctx->stream, &what) != 0) {
+        (void)hipGetLastError();
+        return fail(ctx, EWPT_ERR_HIP, what);
+    }
+    if ((rc = push_view(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "init_thermal sync");
+    ctx->has_state = true;
+    return EWPT_OK;
+}
+
 int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p, const double *din, double *dout, void *stream)
 {
     if (!lattice_ok(lat) || !params_ok(p) || !din || !dout || din == dout) return EWPT_ERR_INVAL;

@@ -46,6 +46,13 @@ void mode_table(unsigned long long seed, int nmodes, int kmax, double amp, doubl
 hipError_t launch_fill_modes(double *y, const ewpt_lattice &L, const double *dmodes, int nmodes, double vev,
                              hipStream_t st);
 
+// ewpt_ic.hip -- bubble and thermal initial conditions
+hipError_t launch_add_bubbles(double *y, const ewpt_lattice &L, const double *dbub, int nb, int profile, double mH,
+                              hipStream_t st);
+// returns 0, or -1 with *what naming the failed call; work holds 3*N^3 complex doubles
+int thermal_state(double *y, const ewpt_lattice &L, double T, double meff_sq, unsigned long long seed, int profile,
+                  void *work, hipStream_t st, const char **what);
+
 inline Phys phys_of(const ewpt_params &p)
 {
     return Phys{p.gw, p.gy, p.gp2, p.beta_w, p.beta_y, p.vev, p.lambda, p.jex, p.gamma};

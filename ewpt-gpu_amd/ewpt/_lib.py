@@ -31,7 +31,7 @@ EXPORTS = (
     "ewpt_create", "ewpt_destroy", "ewpt_set_state", "ewpt_get_state", "ewpt_rhs",
     "ewpt_rhs_device", "ewpt_step", "ewpt_step_timed", "ewpt_energies", "ewpt_gauss",
     "ewpt_set_variant", "ewpt_get_variant", "ewpt_device_bytes", "ewpt_strerror",
-    "ewpt_last_error", "ewpt_version", "ewpt_init_modes",
+    "ewpt_last_error", "ewpt_version", "ewpt_init_modes", "ewpt_init_bubbles", "ewpt_init_thermal",
 )
 
 
@@ -83,6 +83,8 @@ def load() -> ctypes.CDLL:
         "ewpt_device_bytes": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
         "ewpt_init_modes": [vp, ctypes.c_ulonglong, ip, ip, ctypes.c_double, ctypes.c_double,
                             ctypes.c_double, dp],
+        "ewpt_init_bubbles": [vp, dp, ip, ip, ctypes.c_double],
+        "ewpt_init_thermal": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, ip],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -14,6 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import EwptError, EwptLattice, EwptParams, NCH
+from .ic import profile_id
 
 
 def _dptr(a: np.ndarray):
@@ -91,6 +92,19 @@ class Lattice:
         self._check(self._L.ewpt_init_modes(self._h, ctypes.c_ulonglong(seed), nmodes, kmax, amp, vev, mom_scale,
                                             _dptr(t)), "ewpt_init_modes")
         return t
+
+    def init_bubbles(self, table: np.ndarray, profile="ref", mH: float = 2.0 * 0.129 ** 0.5):
+        """Bubble-nucleation state (main_rk4.jl:5640-5698): zero fields plus the
+        superposed profiles of the rows (ib, jb, kb, p1..p4) of ``table``
+        (``ic.bubble_table``)."""
+        t = np.ascontiguousarray(np.asarray(table, dtype=np.float64).reshape(-1, 7))
+        self._check(self._L.ewpt_init_bubbles(self._h, _dptr(t), len(t), profile_id(profile), mH),
+                    "ewpt_init_bubbles")
+
+    def init_thermal(self, T: float, meff_sq: float, seed: int, profile="ref"):
+        """Thermal state (main_rk4.jl:5349-5638) built on the device."""
+        self._check(self._L.ewpt_init_thermal(self._h, T, meff_sq, ctypes.c_ulonglong(seed), profile_id(profile)),
+                    "ewpt_init_thermal")
 
     def get_state(self) -> np.ndarray:
         out = np.empty(self.shape)

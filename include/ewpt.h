@@ -78,6 +78,38 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
 int ewpt_init_modes(ewpt_ctx *ctx, unsigned long long seed, int nmodes, int kmax, double amp,
                     double vev, double mom_scale, double *modes_out);
 
+/* Bubble-nucleation initial condition, replacing the initializer! launches of
+ * the bubble branch of run() (kernel main_rk4.jl:4930-4946, driver
+ * 5640-5698).  Zeroes the whole state, then adds rmag_b(x)*(p1,p2,p3,p4) to
+ * phi for every bubble b.  bubbles holds nb rows of 7 doubles
+ * (ib, jb, kb, p1, p2, p3, p4); the caller draws centres and orientations
+ * (gen_phi, convenients.jl:8-14), as the reference driver does on the host.
+ *   profile 0 (reference): ib,jb,kb are 1-based sub-lattice coordinates and
+ *     rmag = (1+a^2) e^{-mH r/sqrt2} / (1 + a^2 e^{-sqrt2 mH r}), a = sqrt2-1,
+ *     with r^2 = sum_d sin^2(rk_d ((i_d-1) - (ib_d-0.5)) dx)/rk_d^2,
+ *     rk_d = pi/((N_d-1) dx), i_d 1-based (main_rk4.jl:4935-4936);
+ *   profile 1 (tanh wall): ib,jb,kb are real 0-based positions and
+ *     rmag = (1 - tanh((r - 3/mH) mH/2))/2 with r the periodic
+ *     minimum-image distance.
+ * 0 <= nb <= 2^20; mH > 0. */
+int ewpt_init_bubbles(ewpt_ctx *ctx, const double *bubbles, int nb, int profile, double mH);
+
+/* Thermal initial condition, replacing the thermal branch of run()
+ * (thermal_initializer_gauge! / thermal_initializer_phi!,
+ * main_rk4.jl:5029-5152, driver 5349-5638).  Zeroes the state, then for each
+ * gauge field (W^1, W^2, W^3, Y) and each Higgs component builds Gaussian
+ * modes weighted by sqrt(n/(omega vol)), n = 1/(exp(omega/T)-1),
+ * vol = N^3 dx^3 (gauge: transverse polarisations, omega = |k|; Higgs:
+ * omega = sqrt(|k|^2 + meff_sq)), and stores the real part of their
+ * unnormalised inverse FFT.  The normal draws come from a counter-based
+ * generator keyed by (seed, stream, mode index), so a CPU can rebuild the
+ * same state.  profile 0 keeps the reference's mode numbering (array index m
+ * carries the lattice momentum of index m+1, main_rk4.jl:5040-5046) and leaves
+ * Gamma = Sigma = 0; profile 1 uses the signed index and sets Gamma^a = div W^a,
+ * Sigma = div Y.  Momenta stay zero, so the Gauss law holds with E = 0.
+ * Needs T > 0 and finite meff_sq. */
+int ewpt_init_thermal(ewpt_ctx *ctx, double T, double meff_sq, unsigned long long seed, int profile);
+
 /* One RHS evaluation, host buffers: the @cuda rk4_step!(...) launch
  * (main_rk4.jl:52-661; launch 6155).  in = 36-channel stage state, out =
  * the 36 k/kt arrays in the same channel order (0-15 k = momenta copies,
