@@ -452,6 +452,87 @@ int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13])
     return EWPT_OK;
 }
 
+namespace {
+
+// scratch for the diagnostics: the stored stage derivatives are free between
+// steps (FP64: 20 N^3 doubles each, mixed: 20 N^3 floats = 10 N^3 doubles)
+void *diag_work(ewpt_ctx *c, int which)
+{
+    return mixed(c) ? static_cast<void *>(c->fk[which]) : static_cast<void *>(c->k[which]);
+}
+
+bool theta_ok(double th) { return std::isfinite(th); }
+
+}  // namespace
+
+int ewpt_em_energies(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double out[3])
+{
+    if (!ctx || !out || !params_ok(p) || !theta_ok(theta_w)) return EWPT_ERR_INVAL;
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "em_energies before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
+    HIPCHK(ctx, launch_em_energies(ctx->y, ctx->lat, phys_of(*p), theta_w, ctx->part, ctx->res, ctx->stream),
+           "em energy launch");
+    double r[4];
+    HIPCHK(ctx, hipMemcpyAsync(r, ctx->res, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "em energy copy");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "em energy sync");
+    const double vol = ctx->lat.dx * ctx->lat.dx * ctx->lat.dx;
+    out[0] = 0.5 * r[0] * vol;
+    out[1] = 0.5 * r[1] * vol;
+    out[2] = r[2] * vol;
+    if (r[3] != 0.0) return fail(ctx, EWPT_ERR_NONFINITE, "EM field is NaN/Inf");
+    return EWPT_OK;
+}
+
+int ewpt_em_fields(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double *host_out)
+{
+    if (!ctx || !host_out || !params_ok(p) || !theta_ok(theta_w)) return EWPT_ERR_INVAL;
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "em_fields before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
+    double *w = static_cast<double *>(diag_work(ctx, 0));   // 9 N^3 doubles
+    HIPCHK(ctx, launch_em_fields(ctx->y, ctx->lat, phys_of(*p), theta_w, 7u, w, ctx->stream), "em fields launch");
+    HIPCHK(ctx, hipMemcpyAsync(host_out, w, 9 * ctx->n3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+           "em fields copy");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "em fields sync");
+    return EWPT_OK;
+}
+
+int ewpt_spectrum(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int field, const int cut[3], int maxbins,
+                  double *k_out, double *p_out, int *nbins)
+{
+    if (!ctx || !params_ok(p) || !theta_ok(theta_w) || (field != 0 && field != 1) || !cut || !nbins ||
+        (!k_out) != (!p_out))
+        return EWPT_ERR_INVAL;
+    for (int d = 0; d < 3; d++)
+        if (cut[d] < 1 || cut[d] > 4096) return fail(ctx, EWPT_ERR_INVAL, "spectrum: cut out of range");
+    const char *what = "";
+    if (!k_out) {   // size query only
+        int nb = 0;
+        shell_spectrum(nullptr, ctx->lat, cut, nullptr, nullptr, 0, nullptr, nullptr, &nb, &what);
+        *nbins = nb;
+        return EWPT_OK;
+    }
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "spectrum before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
+    double *f = static_cast<double *>(diag_work(ctx, 1));   // 3 N^3 doubles
+    HIPCHK(ctx, launch_em_fields(ctx->y, ctx->lat, phys_of(*p), theta_w, field == 0 ? 1u : 2u, f, ctx->stream),
+           "spectrum field launch");
+    const int r = shell_spectrum(f, ctx->lat, cut, diag_work(ctx, 0), ctx->stream, maxbins, k_out, p_out, nbins, &what);
+    if (r == -2) return fail(ctx, EWPT_ERR_INVAL, what);
+    if (r != 0) {
+        (void)hipGetLastError();
+        return fail(ctx, EWPT_ERR_HIP, what);
+    }
+    for (int b = 0; b < *nbins; b++)
+        if (!std::isfinite(p_out[b])) return fail(ctx, EWPT_ERR_NONFINITE, "spectrum is NaN/Inf");
+    return EWPT_OK;
+}
+
 int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8])
 {
     if (!ctx || !out || !params_ok(p)) return EWPT_ERR_INVAL;

@@ -270,6 +270,58 @@ __global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3
     block_reduce<8>(v, 0u, part);
 }
 
+// Electromagnetic fields (em_point): the channel groups selected by mask
+// (bit 0: B, bit 1: B_2, bit 2: A) are written consecutively to out, three
+// channels per group.
+__global__ void __launch_bounds__(BX *BY) em_fields_kernel(const double *y, long n3, Geo g, Phys P, double sw,
+                                                           double cw, unsigned mask, double *out)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    StageBufs<double> b{};
+    b.y = y;
+    b.n3 = n3;
+    SiteIn<double> s;
+    gather_site<1>(b, g, i, j, k, 0.0, s);
+    double F[3][3];
+    em_point(s, P, sw, cw, F[0], F[1], F[2]);
+    const long q = i + (long)g.nx * (j + (long)g.ny * k);
+    int o = 0;
+#pragma unroll
+    for (int grp = 0; grp < 3; grp++) {
+        if (!(mask >> grp & 1u)) continue;
+#pragma unroll
+        for (int c = 0; c < 3; c++) out[(long)(o + c) * n3 + q] = F[grp][c];
+        o += 3;
+    }
+}
+
+// slots: 0 sum |B|^2, 1 sum |B_2|^2, 2 sum B_2 . A, 3 count of non-finite values
+__global__ void __launch_bounds__(BX *BY) em_density(const double *y, long n3, Geo g, Phys P, double sw, double cw,
+                                                     double *part)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int k = blockIdx.z;
+    double v[4] = {0, 0, 0, 0};
+    if (i < g.nx && j < g.ny) {
+        StageBufs<double> b{};
+        b.y = y;
+        b.n3 = n3;
+        SiteIn<double> s;
+        gather_site<1>(b, g, i, j, k, 0.0, s);
+        double Bv[3], B2[3], A[3];
+        em_point(s, P, sw, cw, Bv, B2, A);
+        v[0] = Bv[0] * Bv[0] + Bv[1] * Bv[1] + Bv[2] * Bv[2];
+        v[1] = B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2];
+        v[2] = B2[0] * A[0] + B2[1] * A[1] + B2[2] * A[2];
+        v[3] = (double)(!isfinite(v[0]) + !isfinite(v[1]) + !isfinite(v[2]));
+    }
+    block_reduce<4>(v, 0u, part);
+}
+
 // Sum (or min) nblk partials of slot q in a fixed order; one block per slot.
 __global__ void __launch_bounds__(256) finish_reduce(const double *part, long nblk, unsigned minmask, double *res)
 {
@@ -347,6 +399,25 @@ hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, d
     const long n3 = (long)L.nx * L.ny * L.nz;
     gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
     finish_reduce<<<8, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_fields(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, unsigned mask,
+                            double *out, hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    const long n3 = (long)L.nx * L.ny * L.nz;
+    em_fields_kernel<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), mask, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_energies(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, double *part,
+                              double *res, hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    const long n3 = (long)L.nx * L.ny * L.nz;
+    em_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
+    finish_reduce<<<4, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
     return hipGetLastError();
 }
 

@@ -21,6 +21,22 @@ hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P
 hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
                         hipStream_t st);
 
+// EM diagnostics (em_point): fields of the groups in mask (bit 0 B, 1 B_2,
+// 2 A) to out; energy sums 0.5|B|^2, 0.5|B_2|^2, B_2.A (unscaled) to res[0..2],
+// non-finite count to res[3]
+hipError_t launch_em_fields(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, unsigned mask,
+                            double *out, hipStream_t st);
+hipError_t launch_em_energies(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, double *part,
+                              double *res, hipStream_t st);
+
+// ewpt_spectrum.hip -- shell-averaged power spectrum of a 3-component field
+// (3 contiguous real N^3 channels in `field`); cwork holds 3 (nx/2+1) ny nz
+// complex doubles.  Returns 0 or -1 with *what naming the failed call.
+int shell_spectrum(const double *field, const ewpt_lattice &L, const int cut[3], void *cwork, hipStream_t st,
+                   int maxbins, double *k_out, double *p_out, int *nbins, const char **what);
+// number of distinct integer |K|^2 of an n0*n1*n2 lattice (Kc_bin_nums)
+long distinct_k2_count(int n0, int n1, int n2);
+
 // ewpt_brick.hip -- LDS-tiled fused stage kernels, variants 1..5
 int brick_variant_count();
 bool brick_exists(int variant);

@@ -236,3 +236,44 @@ __device__ __forceinline__ void rhs_point(const SiteIn<R> &s, const Phys &P, R o
 }
 
 }  // namespace ewpt
+
+namespace ewpt {
+
+// Electromagnetic fields at one site (compute_Bx!/By!/Bz!, compute_B_2_*!,
+// compute_Ax!/Ay!/Az!, main_rk4.jl:2303-2822, 3012-3528, 3718-4250):
+//   n^a   = (-2(p1 p3 + p2 p4), 2(p2 p3 - p1 p4), -p1^2 - p2^2 + p3^2 + p4^2)/|phi|^2,
+//           0 where |phi| = 0 (2459-2467)
+//   B_i   = -(sin th n^a F^a_jk + cos th FY_jk + 4 sin th/(gw v^2) X_i)
+//   B2_i  = the same with |phi|^4 in place of v^2; 0 where |phi| = 0
+//   A_i   = sin th n^a W^a_i + cos th Y_i
+//   X_i   = D_j p1 D_k p2 - D_j p2 D_k p1 + D_j p3 D_k p4 - D_j p4 D_k p3,
+// (i, j, k) cyclic, F^a_jk = B[a][i] and FY_jk = BY[i] of field_strengths.
+template <typename R>
+__device__ __forceinline__ void em_point(const SiteIn<R> &s, const Phys &P, R sw, R cw, R Bem[3], R B2[3], R A[3])
+{
+    R D[4][3];
+    cov_derivs(s, R(P.gw), R(P.gy), D);
+    R B[3][3], BY[3];
+    field_strengths(s, R(P.gw), B, BY);
+    const R p1 = s.c[0], p2 = s.c[1], p3 = s.c[2], p4 = s.c[3];
+    const R mag = p1 * p1 + p2 * p2 + p3 * p3 + p4 * p4;
+    R n[3] = {R(0), R(0), R(0)};
+    if (mag != R(0)) {
+        n[0] = R(-2) * (p1 * p3 + p2 * p4) / mag;
+        n[1] = R(2) * (p2 * p3 - p1 * p4) / mag;
+        n[2] = (-p1 * p1 - p2 * p2 + p3 * p3 + p4 * p4) / mag;
+    }
+    const R c1 = R(4) * sw / (R(P.gw) * R(P.vev) * R(P.vev));
+    const R c2 = mag != R(0) ? R(4) * sw / (R(P.gw) * mag * mag) : R(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        const R X = D[0][j] * D[1][k] - D[1][j] * D[0][k] + D[2][j] * D[3][k] - D[3][j] * D[2][k];
+        const R F = sw * (n[0] * B[0][i] + n[1] * B[1][i] + n[2] * B[2][i]) + cw * BY[i];
+        Bem[i] = -(F + c1 * X);
+        B2[i] = mag != R(0) ? -(F + c2 * X) : R(0);
+        A[i] = sw * (n[0] * s.c[CH_W + i] + n[1] * s.c[CH_W + 3 + i] + n[2] * s.c[CH_W + 6 + i]) + cw * s.c[CH_Y + i];
+    }
+}
+
+}  // namespace ewpt

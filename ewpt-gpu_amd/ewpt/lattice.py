@@ -16,6 +16,9 @@ from . import _lib
 from ._lib import EwptError, EwptLattice, EwptParams, NCH
 from .ic import profile_id
 
+# weak mixing angle of the reference, asin(sqrt(0.22)) (parameters.jl:17, 26)
+THETA_W = float(np.arcsin(np.sqrt(0.22)))
+
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
@@ -141,6 +144,45 @@ class Lattice:
         P = as_params(p)
         self._check(self._L.ewpt_energies(self._h, ctypes.byref(P), _dptr(out)), "ewpt_energies")
         return out
+
+    def em_energies(self, p, theta_w: float = THETA_W) -> np.ndarray:
+        """Energy-table columns 8, 9, 13: 0.5 sum|B|^2 dx^3, 0.5 sum|B_2|^2 dx^3,
+        sum B_2.A dx^3 (main_rk4.jl:6113-6114, 6901)."""
+        out = np.zeros(3)
+        P = as_params(p)
+        self._check(self._L.ewpt_em_energies(self._h, ctypes.byref(P), float(theta_w), _dptr(out)),
+                    "ewpt_em_energies")
+        return out
+
+    def energy_row(self, p, theta_w: float = THETA_W) -> np.ndarray:
+        """The full 13-column row of total_energies (main_rk4.jl:6104-6118)."""
+        row = self.energies(p)
+        row[[7, 8, 12]] = self.em_energies(p, theta_w)
+        return row
+
+    def em_fields(self, p, theta_w: float = THETA_W) -> np.ndarray:
+        """(9, nz, ny, nx): Bx, By, Bz, B2x, B2y, B2z, Ax, Ay, Az."""
+        out = np.empty((9, self.nz, self.ny, self.nx))
+        P = as_params(p)
+        self._check(self._L.ewpt_em_fields(self._h, ctypes.byref(P), float(theta_w), _dptr(out)), "ewpt_em_fields")
+        return out
+
+    def spectrum(self, p, theta_w: float = THETA_W, field: str = "B", cut=None):
+        """Shell spectrum (convolve_1d, spec_routines.jl:47-77) of B or B_2:
+        returns (|K|, mean power) for the first Kc_bin_nums(cut) shells;
+        cut defaults to spec_cut = (N/4, N/4, N/4) of parameters.jl:78."""
+        if field not in ("B", "B2"):
+            raise ValueError("field must be 'B' or 'B2'")
+        c = (ctypes.c_int * 3)(*((self.nx // 4, self.ny // 4, self.nz // 4) if cut is None else cut))
+        P = as_params(p)
+        nb = ctypes.c_int(0)
+        self._check(self._L.ewpt_spectrum(self._h, ctypes.byref(P), float(theta_w), int(field == "B2"), c, 0,
+                                          None, None, ctypes.byref(nb)), "ewpt_spectrum")
+        k = np.zeros(nb.value)
+        pw = np.zeros(nb.value)
+        self._check(self._L.ewpt_spectrum(self._h, ctypes.byref(P), float(theta_w), int(field == "B2"), c,
+                                          nb.value, _dptr(k), _dptr(pw), ctypes.byref(nb)), "ewpt_spectrum")
+        return k, pw
 
     def gauss(self, p) -> np.ndarray:
         """Gauss-law residual norms G^1..3, G^Y, C^1..3, C^Y."""

@@ -51,6 +51,10 @@ class Parameters:
         return math.asin(math.sqrt(self.sin2_thetaW))
 
     @property
+    def theta_w(self) -> float:
+        return self.Wangle                              # parameters.jl:26
+
+    @property
     def dt(self) -> float:
         return self.dx / self.dt_div                   # parameters.jl:23
 

@@ -135,11 +135,42 @@ int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps,
 
 /* Energy table in the reference's column order (main_rk4.jl:6104-6116):
  * out[0..6] = PE, KE, GE, EEW, MEW, EEY, MEY (sum * dx^3, FP64),
- * out[7], out[8] = EM magnetic energies (B, B_2 normalisations; 0 until the
- * B-field kernel runs), out[9] = min|phi|, out[10] = 0, out[11] = mean|phi|,
- * out[12] = 0 (helicity not computed).
+ * out[9] = min|phi|, out[10] = 0, out[11] = mean|phi|.  The EM columns
+ * out[7], out[8], out[12] are left 0: ewpt_em_energies computes them (they
+ * need the weak mixing angle).
  * Returns EWPT_ERR_NONFINITE (after filling out) if any density is NaN/Inf. */
 int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13]);
+
+/* Electromagnetic columns of the energy table, FP64:
+ *   out[0] = column 8:  0.5 sum |B|^2 dx^3   (main_rk4.jl:5830-5868, 6113)
+ *   out[1] = column 9:  0.5 sum |B_2|^2 dx^3 (5887-5926, 6114)
+ *   out[2] = column 13: sum B_2 . A dx^3     (6901; the reference's first
+ *            snapshot, 6118, omits the dx^3)
+ * with B from compute_Bx!/By!/Bz! (2303-2822), B_2 from compute_B_2_x/y/z!
+ * (3718-4250: |phi|^4 in place of vev^2, 0 where phi = 0) and A from
+ * compute_Ax!/Ay!/Az! (3012-3528).  theta_w is the weak mixing angle the
+ * reference passes to these kernels (asin(sqrt(0.22)), parameters.jl:17,26).
+ * Returns EWPT_ERR_NONFINITE if any field value is NaN/Inf. */
+int ewpt_em_energies(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double out[3]);
+
+/* The EM fields themselves, for snapshots (phys_stack, main_rk4.jl:5832-6056):
+ * host_out = 9 N^3 doubles, channels Bx, By, Bz, B2x, B2y, B2z, Ax, Ay, Az,
+ * each x fastest. */
+int ewpt_em_fields(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double *host_out);
+
+/* Shell-averaged magnetic power spectrum: the reference's fft of B_x, B_y,
+ * B_z (main_rk4.jl:5831-5869) followed by convolve_1d(|B_x|^2 + |B_y|^2 +
+ * |B_z|^2, Nx, Ny, Nz, cut...) (spec_routines.jl:47-77, called at
+ * main_rk4.jl:6066-6069 and 7069-7072).  field 0 = B, 1 = B_2.  The shells are
+ * the first Kc_bin_nums(cut[0], cut[1], cut[2]) distinct |K| of the lattice
+ * (K the signed integer mode index, spec_routines.jl:5-43); the reference's
+ * cut is spec_cut = (N/4, N/4, N/4) (parameters.jl:78).  On return
+ * k_out[b] = |K| and p_out[b] = the mean of sum_c |B_c(K)|^2 (unnormalised
+ * forward DFT) over the lattice points of shell b, b < *nbins.  With
+ * k_out = p_out = NULL only *nbins is set (no state needed).  Fails with
+ * EWPT_ERR_INVAL when maxbins < *nbins. */
+int ewpt_spectrum(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int field, const int cut[3],
+                  int maxbins, double *k_out, double *p_out, int *nbins);
 
 /* Gauss-law residual norms, sqrt(sum r^2 dx^3):
  * out[0..2] G^a = div E^a - rho^a, out[3] G^Y = div E^Y - rho^Y,

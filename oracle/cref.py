@@ -2,7 +2,7 @@
 
 Wraps ``libewpt_oracle.so`` built by ``oracle/Makefile`` from
 ``ewpt_oracle.c``, which restates gpu-kernels-rk4/main_rk4.jl:52-661 (RHS),
-6146-6441 (RK4 loop) and 1334-2301 (energy densities).
+6146-6441 (RK4 loop), 1334-2301 (energy densities) and 2303-4250 (EM fields).
 """
 from __future__ import annotations
 
@@ -46,7 +46,8 @@ def lib():
         L.oracle_rk4.argtypes = [i, i, i, ctypes.c_double, pp, dp, i]
         L.oracle_energies.argtypes = [i, i, i, ctypes.c_double, pp, dp, dp]
         L.oracle_gauss.argtypes = [i, i, i, ctypes.c_double, pp, dp, dp]
-        for f in (L.oracle_rhs, L.oracle_rk4, L.oracle_energies, L.oracle_gauss):
+        L.oracle_em_fields.argtypes = [i, i, i, ctypes.c_double, pp, ctypes.c_double, dp, dp]
+        for f in (L.oracle_rhs, L.oracle_rk4, L.oracle_energies, L.oracle_gauss, L.oracle_em_fields):
             f.restype = None
         _lib = L
     return _lib
@@ -101,4 +102,13 @@ def gauss(state: np.ndarray, p, dx: float) -> np.ndarray:
     out = np.zeros(8)
     P = to_params(p)
     lib().oracle_gauss(nx, ny, nz, dx, ctypes.byref(P), _ptr(state), _ptr(out))
+    return out
+
+
+def em_fields(state: np.ndarray, p, dx: float, theta_w: float) -> np.ndarray:
+    """(9, nz, ny, nx): Bx, By, Bz, B2x, B2y, B2z, Ax, Ay, Az."""
+    nx, ny, nz = _check(state)
+    out = np.zeros((9, nz, ny, nx))
+    P = to_params(p)
+    lib().oracle_em_fields(nx, ny, nz, dx, ctypes.byref(P), float(theta_w), _ptr(state), _ptr(out))
     return out

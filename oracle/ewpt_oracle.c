@@ -612,3 +612,59 @@ void oracle_gauss(int nx, int ny, int nz, double dx, const oracle_params *p,
     }
     free(pl);
 }
+
+/* Electromagnetic fields, literal from the reference kernels:
+ *   compute_Bx!/By!/Bz! (main_rk4.jl:2303-2822): n^a at 2459-2467, B_x at 2469-2471,
+ *     B_y at 2643-2645, B_z at 2817-2819;
+ *   compute_B_2_x!/y!/z! (3718-4250): the same with gw*phi_mag^2 for gw*vev^2
+ *     (phi_mag = |phi|^2) and 0 where phi_mag == 0;
+ *   compute_Ax!/Ay!/Az! (3012-3528; A_x at 3179, A_y at 3352, A_z at 3525):
+ *     A_i = sin(th) n^a W^a_i + cos(th) Y_i.
+ * out: 9 channels Bx, By, Bz, B2x, B2y, B2z, Ax, Ay, Az, each nx*ny*nz. */
+void oracle_em_fields(int nx, int ny, int nz, double dx, const oracle_params *p, double theta_w,
+                      const double *y, double *out)
+{
+    lat_t L = {nx, ny, nz, dx};
+    const long N3 = (long)nx * ny * nz;
+    const double *in = y;
+    const double sw = sin(theta_w), cw = cos(theta_w), gw = p->gw, vev = p->vev;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                site_t s;
+                load_site(&L, p, in, N3, i, j, k, &s);
+                const long c = sidx(&L, i, j, k);
+                const double mag = s.p1 * s.p1 + s.p2 * s.p2 + s.p3 * s.p3 + s.p4 * s.p4;
+                double n1, n2, n3;
+                if (mag == 0.0) {
+                    n1 = 0.0; n2 = 0.0; n3 = 0.0;
+                } else {
+                    n1 = -2. * (s.p1 * s.p3 + s.p2 * s.p4) / mag;
+                    n2 = +2. * (s.p2 * s.p3 - s.p1 * s.p4) / mag;
+                    n3 = (-s.p1 * s.p1 - s.p2 * s.p2 + s.p3 * s.p3 + s.p4 * s.p4) / mag;
+                }
+                /* D[b][dir]: dir 0 = x, 1 = y, 2 = z */
+                const double Xx = s.D[0][1] * s.D[1][2] - s.D[1][1] * s.D[0][2] + s.D[2][1] * s.D[3][2] - s.D[3][1] * s.D[2][2];
+                const double Xy = s.D[0][2] * s.D[1][0] - s.D[1][2] * s.D[0][0] + s.D[2][2] * s.D[3][0] - s.D[3][2] * s.D[2][0];
+                const double Xz = s.D[0][0] * s.D[1][1] - s.D[1][0] * s.D[0][1] + s.D[2][0] * s.D[3][1] - s.D[3][0] * s.D[2][1];
+                const double Fx = sw * (n1 * s.W_1_34 + n2 * s.W_2_34 + n3 * s.W_3_34) + cw * s.Y_3_4;
+                const double Fy = sw * (n1 * (-s.W_1_24) + n2 * (-s.W_2_24) + n3 * (-s.W_3_24)) + cw * (-s.Y_2_4);
+                const double Fz = sw * (n1 * (s.W_1_23) + n2 * s.W_2_23 + n3 * s.W_3_23) + cw * s.Y_2_3;
+                const double c1 = 4. * sw / (gw * vev * vev);
+                out[0 * N3 + c] = -(Fx + c1 * Xx);
+                out[1 * N3 + c] = -(Fy + c1 * Xy);
+                out[2 * N3 + c] = -(Fz + c1 * Xz);
+                if (mag == 0.0) {
+                    out[3 * N3 + c] = 0.0; out[4 * N3 + c] = 0.0; out[5 * N3 + c] = 0.0;
+                } else {
+                    const double c2 = 4. * sw / (gw * mag * mag);
+                    out[3 * N3 + c] = -(Fx + c2 * Xx);
+                    out[4 * N3 + c] = -(Fy + c2 * Xy);
+                    out[5 * N3 + c] = -(Fz + c2 * Xz);
+                }
+                out[6 * N3 + c] = sw * (n1 * s.W12 + n2 * s.W22 + n3 * s.W32) + cw * s.Y2;
+                out[7 * N3 + c] = sw * (n1 * s.W13 + n2 * s.W23 + n3 * s.W33) + cw * s.Y3;
+                out[8 * N3 + c] = sw * (n1 * s.W14 + n2 * s.W24 + n3 * s.W34) + cw * s.Y4;
+            }
+}

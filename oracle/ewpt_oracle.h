@@ -44,6 +44,12 @@ void oracle_energies(int nx, int ny, int nz, double dx, const oracle_params *p,
 void oracle_gauss(int nx, int ny, int nz, double dx, const oracle_params *p,
                   const double *y, double out[8]);
 
+/* Electromagnetic fields (compute_Bx!/By!/Bz!, compute_B_2_*!, compute_A*!,
+ * main_rk4.jl:2303-2822, 3012-3530, 3718-4250): out = 9 channels
+ * Bx, By, Bz, B2x, B2y, B2z, Ax, Ay, Az. */
+void oracle_em_fields(int nx, int ny, int nz, double dx, const oracle_params *p, double theta_w,
+                      const double *y, double *out);
+
 #ifdef __cplusplus
 }
 #endif
