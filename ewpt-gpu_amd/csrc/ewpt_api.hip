@@ -500,6 +500,22 @@ int ewpt_em_fields(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double *
     return EWPT_OK;
 }
 
+int ewpt_y_slice(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int j, double *host_out)
+{
+    if (!ctx || !host_out || !params_ok(p) || !theta_ok(theta_w) || j < 0 || j >= ctx->lat.ny)
+        return EWPT_ERR_INVAL;
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "y_slice before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
+    double *w = static_cast<double *>(diag_work(ctx, 0));   // 7 nx nz doubles
+    const size_t n = 7 * (size_t)ctx->lat.nx * ctx->lat.nz;
+    HIPCHK(ctx, launch_y_slice(ctx->y, ctx->lat, phys_of(*p), theta_w, j, w, ctx->stream), "y_slice launch");
+    HIPCHK(ctx, hipMemcpyAsync(host_out, w, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "y_slice copy");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "y_slice sync");
+    return EWPT_OK;
+}
+
 int ewpt_spectrum(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int field, const int cut[3], int maxbins,
                   double *k_out, double *p_out, int *nbins)
 {

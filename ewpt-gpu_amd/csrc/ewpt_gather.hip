@@ -185,10 +185,37 @@ __device__ void block_reduce(double (&v)[NQ], unsigned minmask, double *part)
     }
 }
 
+// Energy densities at one site, the reference's compute_* kernels:
+// e[0] PE (main_rk4.jl:1341-1342), e[1] KE (1659), e[2] GE (1498-1501),
+// e[3] EEW (1817-1820), e[4] MEW (1978-1981), e[5] EEY (2139-2140), e[6] MEY (2298).
+__device__ __forceinline__ void energy_terms(const SiteIn<double> &s, const Phys &P, double e[7])
+{
+    const double r2 = s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3];
+    const double X = r2 - P.vev * P.vev;
+    e[0] = P.lam * (X * X);
+    e[1] = s.c[CH_PI] * s.c[CH_PI] + s.c[CH_PI + 1] * s.c[CH_PI + 1] + s.c[CH_PI + 2] * s.c[CH_PI + 2] +
+           s.c[CH_PI + 3] * s.c[CH_PI + 3];
+    double D[4][3];
+    cov_derivs(s, P.gw, P.gy, D);
+    double ge = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) ge += D[a][0] * D[a][0] + D[a][1] * D[a][1] + D[a][2] * D[a][2];
+    e[2] = ge;
+    double B[3][3], BYv[3];
+    field_strengths(s, P.gw, B, BYv);
+    double ee = 0.0, me = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) ee += s.c[CH_E + q] * s.c[CH_E + q];
+#pragma unroll
+    for (int a = 0; a < 3; a++) me += B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2];
+    e[3] = 0.5 * ee;
+    e[4] = 0.5 * me;
+    e[5] = 0.5 * (s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2]);
+    e[6] = 0.5 * (BYv[0] * BYv[0] + BYv[1] * BYv[1] + BYv[2] * BYv[2]);
+}
+
 // slots: 0 PE, 1 KE, 2 GE, 3 EEW, 4 MEW, 5 EEY, 6 MEY, 7 sum|phi|, 8 min|phi|,
 // 9 count of non-finite densities.
-// Densities: main_rk4.jl:1341-1342 (PE), 1659 (KE), 1498-1501 (GE),
-// 1817-1820 (EEW), 1978-1981 (MEW), 2139-2140 (EEY), 2298 (MEY).
 __global__ void __launch_bounds__(BX *BY) energy_density(const double *y, long n3, Geo g, Phys P, double *part)
 {
     const int i = blockIdx.x * BX + threadIdx.x;
@@ -201,29 +228,8 @@ __global__ void __launch_bounds__(BX *BY) energy_density(const double *y, long n
         b.n3 = n3;
         SiteIn<double> s;
         gather_site<1>(b, g, i, j, k, 0.0, s);
-        const double r2 = s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3];
-        const double X = r2 - P.vev * P.vev;
-        v[0] = P.lam * (X * X);
-        v[1] = s.c[CH_PI] * s.c[CH_PI] + s.c[CH_PI + 1] * s.c[CH_PI + 1] + s.c[CH_PI + 2] * s.c[CH_PI + 2] +
-               s.c[CH_PI + 3] * s.c[CH_PI + 3];
-        double D[4][3];
-        cov_derivs(s, P.gw, P.gy, D);
-        double ge = 0.0;
-#pragma unroll
-        for (int a = 0; a < 4; a++) ge += D[a][0] * D[a][0] + D[a][1] * D[a][1] + D[a][2] * D[a][2];
-        v[2] = ge;
-        double B[3][3], BYv[3];
-        field_strengths(s, P.gw, B, BYv);
-        double ee = 0.0, me = 0.0;
-#pragma unroll
-        for (int q = 0; q < 9; q++) ee += s.c[CH_E + q] * s.c[CH_E + q];
-#pragma unroll
-        for (int a = 0; a < 3; a++) me += B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2];
-        v[3] = 0.5 * ee;
-        v[4] = 0.5 * me;
-        v[5] = 0.5 * (s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2]);
-        v[6] = 0.5 * (BYv[0] * BYv[0] + BYv[1] * BYv[1] + BYv[2] * BYv[2]);
-        const double m = sqrt(r2);
+        energy_terms(s, P, v);
+        const double m = sqrt(s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3]);
         v[7] = m;
         v[8] = m;
         int bad = 0;
@@ -232,6 +238,36 @@ __global__ void __launch_bounds__(BX *BY) energy_density(const double *y, long n
         v[9] = (double)bad;
     }
     block_reduce<10>(v, 1u << 8, part);
+}
+
+// The reference's y-slice snapshot (slices_arr, main_rk4.jl:5700-5930,
+// 6450-6720) on the plane y = j0: per (x, z), times dx^3,
+//   0 PE, 1 KE, 2 GE, 3 MEW + EEW + MEY + EEY, 4 Bx + By + Bz, 5 B2x + B2y + B2z,
+//   6 zero (the seventh slice type is never written).
+// out: 7 planes of nx*nz, x fastest.
+__global__ void __launch_bounds__(256) y_slice_kernel(const double *y, long n3, Geo g, Phys P, double sw, double cw,
+                                                      int j0, double vol, double *out)
+{
+    const long np = (long)g.nx * g.nz;
+    const long q = blockIdx.x * 256L + threadIdx.x;
+    if (q >= np) return;
+    const int i = (int)(q % g.nx), k = (int)(q / g.nx);
+    StageBufs<double> b{};
+    b.y = y;
+    b.n3 = n3;
+    SiteIn<double> s;
+    gather_site<1>(b, g, i, j0, k, 0.0, s);
+    double e[7];
+    energy_terms(s, P, e);
+    double Bv[3], B2[3], A[3];
+    em_point(s, P, sw, cw, Bv, B2, A);
+    out[0 * np + q] = e[0] * vol;
+    out[1 * np + q] = e[1] * vol;
+    out[2 * np + q] = e[2] * vol;
+    out[3 * np + q] = e[4] * vol + e[3] * vol + e[6] * vol + e[5] * vol;
+    out[4 * np + q] = Bv[0] * vol + Bv[1] * vol + Bv[2] * vol;
+    out[5 * np + q] = B2[0] * vol + B2[1] * vol + B2[2] * vol;
+    out[6 * np + q] = 0.0;
 }
 
 // Gauss residuals (squared), slots 0..7 as documented in ewpt.h.
@@ -408,6 +444,16 @@ hipError_t launch_em_fields(const double *y, const ewpt_lattice &L, const Phys &
     const Geo g = make_geo(L);
     const long n3 = (long)L.nx * L.ny * L.nz;
     em_fields_kernel<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), mask, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_y_slice(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, int j0, double *out,
+                          hipStream_t st)
+{
+    const Geo g = make_geo(L);
+    const long n3 = (long)L.nx * L.ny * L.nz, np = (long)L.nx * L.nz;
+    y_slice_kernel<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), j0,
+                                                                 L.dx * L.dx * L.dx, out);
     return hipGetLastError();
 }
 

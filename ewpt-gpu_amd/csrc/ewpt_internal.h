@@ -28,6 +28,9 @@ hipError_t launch_em_fields(const double *y, const ewpt_lattice &L, const Phys &
                             double *out, hipStream_t st);
 hipError_t launch_em_energies(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, double *part,
                               double *res, hipStream_t st);
+// the seven y-slice snapshot planes at y = j0 (7 * nx * nz doubles to out)
+hipError_t launch_y_slice(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, int j0, double *out,
+                          hipStream_t st);
 
 // ewpt_spectrum.hip -- shell-averaged power spectrum of a 3-component field
 // (3 contiguous real N^3 channels in `field`); cwork holds 3 (nx/2+1) ny nz

@@ -32,7 +32,7 @@ EXPORTS = (
     "ewpt_rhs_device", "ewpt_step", "ewpt_step_timed", "ewpt_energies", "ewpt_gauss",
     "ewpt_set_variant", "ewpt_get_variant", "ewpt_device_bytes", "ewpt_strerror",
     "ewpt_last_error", "ewpt_version", "ewpt_init_modes", "ewpt_init_bubbles", "ewpt_init_thermal",
-    "ewpt_em_energies", "ewpt_em_fields", "ewpt_spectrum",
+    "ewpt_em_energies", "ewpt_em_fields", "ewpt_spectrum", "ewpt_y_slice",
 )
 
 
@@ -88,6 +88,7 @@ def load() -> ctypes.CDLL:
         "ewpt_init_thermal": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, ip],
         "ewpt_em_energies": [vp, pp, ctypes.c_double, dp],
         "ewpt_em_fields": [vp, pp, ctypes.c_double, dp],
+        "ewpt_y_slice": [vp, pp, ctypes.c_double, ip, dp],
         "ewpt_spectrum": [vp, pp, ctypes.c_double, ip, ctypes.POINTER(ctypes.c_int), ip, dp, dp,
                           ctypes.POINTER(ctypes.c_int)],
     }

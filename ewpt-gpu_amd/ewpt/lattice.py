@@ -167,6 +167,15 @@ class Lattice:
         self._check(self._L.ewpt_em_fields(self._h, ctypes.byref(P), float(theta_w), _dptr(out)), "ewpt_em_fields")
         return out
 
+    def y_slice(self, p, theta_w: float = THETA_W, j: int | None = None) -> np.ndarray:
+        """(7, nz, nx) snapshot planes at y = j (default ny/2 - 1, the reference's
+        1-based Ny/2): PE, KE, GE, gauge energy, sum B_i, sum B2_i, 0; times dx^3."""
+        j = self.ny // 2 - 1 if j is None else int(j)
+        out = np.empty((7, self.nz, self.nx))
+        P = as_params(p)
+        self._check(self._L.ewpt_y_slice(self._h, ctypes.byref(P), float(theta_w), j, _dptr(out)), "ewpt_y_slice")
+        return out
+
     def spectrum(self, p, theta_w: float = THETA_W, field: str = "B", cut=None):
         """Shell spectrum (convolve_1d, spec_routines.jl:47-77) of B or B_2:
         returns (|K|, mean power) for the first Kc_bin_nums(cut) shells;

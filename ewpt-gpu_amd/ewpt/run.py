@@ -1,0 +1,130 @@
+"""The run() driver of the reference (gpu-kernels-rk4/main_rk4.jl:5174-7081)
+over libewpt: initial condition, evolve loop with the per-step J_ext and damp
+scalars, snapshots of the energy table, y-slices and spectra, and the output
+files (``ewpt.io``).  Plots are not produced.
+
+Command line, the reference's 12 positional ARGS (parameters.jl:14-55)::
+
+    python -m ewpt.run gamma no_bubbles nte dx dt_div N gp2 nsnaps T beta_Y beta_W seed_mult \\
+        [--bubbles] [--profile ref|config] [--precision fp64|mixed] [--out DIR] [--fft]
+
+The reference's entry point runs the thermal IC (run(thermal_init=true),
+main_rk4.jl:7082); ``--bubbles`` selects the bubble branch.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import io
+from ._lib import EWPT_FP64, EWPT_MIXED
+from .ic import bubble_table, profile_id
+from .lattice import Lattice
+from .parameters import J_ext, Parameters, damp
+
+
+def _fft_record(lat: Lattice, p, theta_w: float, path: str):
+    """3d-fft-data file: full forward DFTs of B and B_2 (main_rk4.jl:6861-6868)."""
+    em = lat.em_fields(p, theta_w)
+    with io.Writer(path) as w:
+        for c, name in enumerate(("3D_fft_x", "3D_fft_y", "3D_fft_z", "3D_fft_x2", "3D_fft_y2", "3D_fft_z2")):
+            # em[c] is (nz, ny, nx) x fastest; its transpose is the Julia (Nx, Ny, Nz) array
+            w.add(name, np.fft.fftn(em[c].T))
+
+
+def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int = EWPT_FP64,
+        out_dir: str | None = ".", write_fft: bool = False, nsnaps_fft: int = 50, verbose: bool = True) -> dict:
+    """Evolve one lattice as run(thermal_init) does; returns the outputs and,
+    when out_dir is given, writes data.ewpt (and 3d-fft-data-<it>.ewpt files
+    every dsnaps_fft steps with write_fft)."""
+    N, dx, dt, mH = P.N, P.dx, P.dt, P.mH
+    prof = profile_id(profile)
+    theta_w = P.theta_w
+    dsnaps = P.dsnaps
+    dsnaps_fft = max(1, P.nte // nsnaps_fft)
+    if dsnaps < 1:
+        raise ValueError("nte must be >= nsnaps")
+    lat = Lattice(N, dx, precision=precision)
+    res: dict = {}
+    try:
+        seed = P.base_seed
+        if thermal_init:
+            lat.init_thermal(P.T, P.meff_sq, seed, prof)
+        else:
+            table = bubble_table(N, P.no_bubbles, seed, prof, bub_diam=P.bub_diam)
+            lat.init_bubbles(table, prof, mH)
+            res["bubble_table"] = table
+
+        p0 = P.step_params()
+        energies = np.zeros((P.nsnaps + 1, 13))
+        slices = np.zeros((P.nsnaps + 1, 7, N, N))     # Julia (nsnaps+1, 7, Nx, Nz)
+        energies[0] = lat.energy_row(p0, theta_w)
+        slices[0] = np.transpose(lat.y_slice(p0, theta_w), (0, 2, 1))
+        k0, s0 = lat.spectrum(p0, theta_w)
+        if out_dir and write_fft:
+            _fft_record(lat, p0, theta_w, os.path.join(out_dir, "3d-fft-data-0.ewpt"))
+
+        snp = 0
+        t0 = time.perf_counter()
+        for it in range(1, P.nte + 1):
+            jex = J_ext(it, dt, mH, P.t_sat, thermal_init=thermal_init)
+            gam = damp(it, dt, mH, P.t_sat, P.gamma_inp, thermal_init=thermal_init)
+            p = P.step_params(jex=jex, gamma=gam)
+            lat.step(p, 1)
+            if it % dsnaps == 0 and snp < P.nsnaps:
+                snp += 1
+                energies[snp] = lat.energy_row(p, theta_w)
+                slices[snp] = np.transpose(lat.y_slice(p, theta_w), (0, 2, 1))
+                if verbose:
+                    print(f"it={it} t*mH={it * dt * mH:.3f} E_tot={energies[snp, :7].sum():.9e} "
+                          f"B={energies[snp, 7]:.6e} ({time.perf_counter() - t0:.1f} s)", flush=True)
+            if out_dir and write_fft and it % dsnaps_fft == 0:
+                _fft_record(lat, p, theta_w, os.path.join(out_dir, f"3d-fft-data-{it}.ewpt"))
+        k1, s1 = lat.spectrum(P.step_params(), theta_w)
+        res.update(energies=energies, slices=slices, spectrum=np.stack([np.stack([k0, s0], 1), np.stack([k1, s1], 1)]),
+                   gauss=lat.gauss(P.step_params()))
+        res["final_state"] = lat.get_state() if N <= 64 else None
+    finally:
+        lat.close()
+
+    if out_dir:
+        with io.Writer(os.path.join(out_dir, "data.ewpt")) as w:
+            for name, v in (("Nx", N), ("gp2", P.gp2), ("dx", dx), ("dt", dt), ("nte", P.nte),
+                            ("gamma", P.gamma_inp), ("T", P.T), ("meff_sq", P.meff_sq), ("t_sat_jex", P.t_sat),
+                            ("nsnaps", P.nsnaps), ("dsnaps", dsnaps)):
+                w.add(name, v)
+            if not thermal_init:
+                t = res["bubble_table"]
+                w.add("no_bubbles", P.no_bubbles)
+                w.add("bubble_locations", t[:, :3])
+                w.add("Hoft_angles", t[:, 3:])
+            w.add("energies", res["energies"])
+            w.add("y_slices_snaps", res["slices"])
+            w.add("B_linear_fft", res["spectrum"])
+    return res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="run() of the electroweak lattice evolution")
+    ap.add_argument("args", nargs=12, help="the 12 positional ARGS of parameters.jl")
+    ap.add_argument("--bubbles", action="store_true", help="bubble-nucleation IC instead of thermal")
+    ap.add_argument("--profile", choices=("ref", "config"), default="ref")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--gy", type=float, default=None, help="override the derived g_y")
+    ap.add_argument("--out", default=".")
+    ap.add_argument("--fft", action="store_true", help="write 3d-fft-data files")
+    a = ap.parse_args(argv)
+    over = {} if a.gy is None else {"gy": a.gy}
+    P = Parameters.from_args(a.args, **over)
+    os.makedirs(a.out, exist_ok=True)
+    run(P, thermal_init=not a.bubbles, profile=a.profile,
+        precision=EWPT_FP64 if a.precision == "fp64" else EWPT_MIXED, out_dir=a.out, write_fft=a.fft)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

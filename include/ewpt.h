@@ -158,6 +158,14 @@ int ewpt_em_energies(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double
  * each x fastest. */
 int ewpt_em_fields(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double *host_out);
 
+/* The y-slice snapshot of the reference (slices_arr[snp, 1:7, :, :],
+ * main_rk4.jl:5700-5930, 6450-6720) on the plane y = j (0-based; the
+ * reference takes its 1-based Ny/2, i.e. j = ny/2 - 1): host_out = 7 planes of
+ * nx*nz doubles, x fastest, each value times dx^3:
+ *   0 PE, 1 KE, 2 GE densities, 3 MEW + EEW + MEY + EEY, 4 Bx + By + Bz,
+ *   5 B2x + B2y + B2z, 6 zero (the reference never fills the seventh type). */
+int ewpt_y_slice(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int j, double *host_out);
+
 /* Shell-averaged magnetic power spectrum: the reference's fft of B_x, B_y,
  * B_z (main_rk4.jl:5831-5869) followed by convolve_1d(|B_x|^2 + |B_y|^2 +
  * |B_z|^2, Nx, Ny, Nz, cut...) (spec_routines.jl:47-77, called at
