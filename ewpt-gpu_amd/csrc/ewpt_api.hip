@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -607,6 +609,371 @@ int ewpt_device_bytes(ewpt_ctx *ctx, unsigned long long *bytes)
 {
     if (!ctx || !bytes) return EWPT_ERR_INVAL;
     *bytes = ctx->bytes;
+    return EWPT_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+}  // extern "C"
+
+struct ewpt_batch {
+    int n = 0, device = 0, precision = EWPT_FP64;
+    std::vector<ewpt_ctx *> ctx;
+    int variant = 0;        // 0: first candidate that tiles every lattice
+    BatchPlan *plan = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_beta = -1;    // beta flavour the captured graph was built for
+    std::string err;
+};
+
+namespace {
+
+int bfail(ewpt_batch *b, int code, const char *what, hipError_t e = hipSuccess)
+{
+    char buf[512];
+    if (e != hipSuccess) std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else std::snprintf(buf, sizeof buf, "%s", what);
+    b->err = buf;
+    return code;
+}
+
+void drop_graph(ewpt_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_beta = -1;
+}
+
+void drop_plan(ewpt_batch *b)
+{
+    drop_graph(b);
+    batch_plan_destroy(b->plan);
+    b->plan = nullptr;
+}
+
+// candidate variants in preference order (measured single-lattice ranking)
+std::vector<int> batch_candidates(bool f32)
+{
+    std::vector<int> v = f32 ? std::vector<int>{210, 211, 110, 111, 10, 11}
+                             : std::vector<int>{211, 210, 111, 110, 11, 10, 203, 103, 3, 2};
+    const size_t n = v.size();
+    for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms
+    return v;
+}
+
+bool batch_fits_all(const ewpt_batch *b, int v)
+{
+    for (const ewpt_ctx *c : b->ctx)
+        if (!batch_variant_fits(v, b->precision == EWPT_MIXED, c->lat)) return false;
+    return true;
+}
+
+int batch_resolved(const ewpt_batch *b)
+{
+    if (b->variant) return b->variant;
+    for (int v : batch_candidates(b->precision == EWPT_MIXED))
+        if (batch_fits_all(b, v)) return v;
+    return -1;
+}
+
+int ensure_plan(ewpt_batch *b)
+{
+    if (b->plan) return EWPT_OK;
+    const int v = batch_resolved(b);
+    if (v < 0) return bfail(b, EWPT_ERR_UNSUPPORTED, "no batch variant tiles every lattice");
+    std::vector<ewpt_lattice> lats;
+    for (const ewpt_ctx *c : b->ctx) lats.push_back(c->lat);
+    const char *what = "";
+    b->plan = batch_plan_create(v, b->precision == EWPT_MIXED, lats, b->device, &what);
+    if (!b->plan) return bfail(b, EWPT_ERR_HIP, what);
+    return EWPT_OK;
+}
+
+template <typename T>
+void batch_lat(const ewpt_ctx *c, const ewpt_params &p, BatchLat<T> &L)
+{
+    std::memset(&L, 0, sizeof L);
+    L.L = c->lat;
+    L.P = phys_of(p);
+    L.dt = p.dt;
+    for (int par = 0; par < 2; par++)
+        for (int s = 1; s <= 4; s++) {
+            StageBufs<T> &sb = L.b[par][s - 1];
+            sb.n3 = c->n3;
+            if constexpr (std::is_same_v<T, float>) {
+                sb.y = par ? c->fhi_n : c->fhi;
+                sb.ylo = par ? c->flo_n : c->flo;
+                sb.ynew = par ? c->fhi : c->fhi_n;
+                sb.ylo_new = par ? c->flo : c->flo_n;
+                sb.k1 = c->fk[0]; sb.k2 = c->fk[1]; sb.k3 = c->fk[2];
+                sb.kout = s < 4 ? c->fk[s - 1] : nullptr;
+            } else {
+                sb.y = par ? c->ynext : c->y;
+                sb.ynew = par ? c->y : c->ynext;
+                sb.k1 = c->k[0]; sb.k2 = c->k[1]; sb.k3 = c->k[2];
+                sb.kout = s < 4 ? c->k[s - 1] : nullptr;
+            }
+        }
+}
+
+// upload the descriptors for params p (one per lattice)
+int batch_upload(ewpt_batch *b, const ewpt_params *p)
+{
+    int r;
+    if (b->precision == EWPT_MIXED) {
+        std::vector<BatchLat<float>> v(b->n);
+        for (int i = 0; i < b->n; i++) batch_lat(b->ctx[i], p[i], v[i]);
+        r = batch_plan_set(b->plan, v.data(), b->stream);
+    } else {
+        std::vector<BatchLat<double>> v(b->n);
+        for (int i = 0; i < b->n; i++) batch_lat(b->ctx[i], p[i], v[i]);
+        r = batch_plan_set(b->plan, v.data(), b->stream);
+    }
+    return r ? bfail(b, EWPT_ERR_HIP, "batch descriptor upload") : EWPT_OK;
+}
+
+int batch_launch_step(ewpt_batch *b, int parity, bool beta)
+{
+    for (int s = 1; s <= 4; s++) {
+        const hipError_t e = batch_plan_launch(b->plan, parity, s, beta, b->stream);
+        if (e != hipSuccess) return bfail(b, EWPT_ERR_HIP, "batch stage launch", e);
+    }
+    return EWPT_OK;
+}
+
+int ensure_graph(ewpt_batch *b, bool beta)
+{
+    if (b->exec && b->graph_beta == (int)beta) return EWPT_OK;
+    drop_graph(b);
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return bfail(b, EWPT_ERR_HIP, "graph capture begin", e);
+    int rc = batch_launch_step(b, 0, beta);
+    if (rc == EWPT_OK) rc = batch_launch_step(b, 1, beta);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, EWPT_ERR_HIP, "graph capture end", e);
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        return bfail(b, EWPT_ERR_HIP, "graph instantiate", e);
+    }
+    b->graph_beta = beta;
+    return EWPT_OK;
+}
+
+// after nsteps batched steps: an odd count leaves the update in the other buffers
+void batch_commit(ewpt_batch *b, int nsteps)
+{
+    for (ewpt_ctx *c : b->ctx) {
+        if (nsteps % 2) {
+            if (mixed(c)) { std::swap(c->fhi, c->fhi_n); std::swap(c->flo, c->flo_n); }
+            else std::swap(c->y, c->ynext);
+        }
+        if (nsteps > 0 && mixed(c)) c->view_valid = false;
+    }
+}
+
+int batch_check(ewpt_batch *b, const ewpt_params *p, int nsteps)
+{
+    if (!p || nsteps < 0) return EWPT_ERR_INVAL;
+    for (int i = 0; i < b->n; i++) {
+        if (!params_ok(&p[i])) return bfail(b, EWPT_ERR_INVAL, "batch: invalid parameters");
+        if (!b->ctx[i]->has_state) return bfail(b, EWPT_ERR_STATE, "batch: a lattice has no state");
+    }
+    if (hipSetDevice(b->device) != hipSuccess) return bfail(b, EWPT_ERR_HIP, "hipSetDevice");
+    return ensure_plan(b);
+}
+
+bool any_beta(const ewpt_batch *b, const ewpt_params *p)
+{
+    for (int i = 0; i < b->n; i++)
+        if (p[i].beta_w != 0.0 || p[i].beta_y != 0.0) return true;
+    return false;
+}
+
+// nsteps steps: pairs through the captured two-step graph, an odd last step eagerly
+int batch_run(ewpt_batch *b, const ewpt_params *p, int nsteps, bool use_graph)
+{
+    const bool beta = any_beta(b, p);
+    int rc = batch_upload(b, p);
+    if (rc) return rc;
+    if (use_graph && nsteps >= 2 && (rc = ensure_graph(b, beta))) return rc;
+    for (int it = 0; it + 1 < nsteps; it += 2) {
+        if (use_graph) {
+            const hipError_t e = hipGraphLaunch(b->exec, b->stream);
+            if (e != hipSuccess) return bfail(b, EWPT_ERR_HIP, "graph launch", e);
+        } else if ((rc = batch_launch_step(b, 0, beta)) || (rc = batch_launch_step(b, 1, beta))) {
+            return rc;
+        }
+    }
+    if (nsteps % 2 && (rc = batch_launch_step(b, 0, beta))) return rc;
+    return EWPT_OK;
+}
+
+std::string batch_key(const ewpt_batch *b)
+{
+    std::vector<std::string> sh;
+    for (const ewpt_ctx *c : b->ctx)
+        sh.push_back(std::to_string(c->lat.nx) + "x" + std::to_string(c->lat.ny) + "x" + std::to_string(c->lat.nz));
+    std::sort(sh.begin(), sh.end());
+    std::string k = std::to_string(b->precision) + ":" + std::to_string(b->device);
+    for (const std::string &x : sh) k += "," + x;
+    return k;
+}
+
+std::map<std::string, int> &tune_cache()
+{
+    static std::map<std::string, int> m;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ewpt_batch_create(ewpt_batch **out, ewpt_ctx *const *ctxs, int n)
+{
+    if (!out || !ctxs || n < 1) return EWPT_ERR_INVAL;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return EWPT_ERR_INVAL;
+        if (ctxs[i]->device != ctxs[0]->device || ctxs[i]->precision != ctxs[0]->precision) return EWPT_ERR_INVAL;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return EWPT_ERR_INVAL;
+    }
+    ewpt_batch *b = new (std::nothrow) ewpt_batch();
+    if (!b) return EWPT_ERR_OOM;
+    b->n = n;
+    b->device = ctxs[0]->device;
+    b->precision = ctxs[0]->precision;
+    b->ctx.assign(ctxs, ctxs + n);
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return EWPT_ERR_HIP;
+    }
+    if (batch_resolved(b) < 0) {
+        (void)hipStreamDestroy(b->stream);
+        delete b;
+        return EWPT_ERR_UNSUPPORTED;
+    }
+    *out = b;
+    return EWPT_OK;
+}
+
+int ewpt_batch_destroy(ewpt_batch *b)
+{
+    if (!b) return EWPT_ERR_INVAL;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_plan(b);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return EWPT_OK;
+}
+
+const char *ewpt_batch_last_error(ewpt_batch *b) { return b ? b->err.c_str() : ""; }
+
+int ewpt_batch_set_variant(ewpt_batch *b, int variant)
+{
+    if (!b) return EWPT_ERR_INVAL;
+    if (variant != 0 && !batch_fits_all(b, variant))
+        return bfail(b, EWPT_ERR_UNSUPPORTED, "batch variant does not tile every lattice");
+    if (variant != b->variant) {
+        if (hipSetDevice(b->device) == hipSuccess && b->stream) (void)hipStreamSynchronize(b->stream);
+        drop_plan(b);
+        b->variant = variant;
+    }
+    return EWPT_OK;
+}
+
+int ewpt_batch_get_variant(ewpt_batch *b, int *variant)
+{
+    if (!b || !variant) return EWPT_ERR_INVAL;
+    *variant = batch_resolved(b);
+    return EWPT_OK;
+}
+
+int ewpt_batch_step(ewpt_batch *b, const ewpt_params *p, int nsteps)
+{
+    if (!b) return EWPT_ERR_INVAL;
+    int rc = batch_check(b, p, nsteps);
+    if (rc) return rc;
+    rc = batch_run(b, p, nsteps, true);
+    const hipError_t e = hipStreamSynchronize(b->stream);
+    if (rc) return rc;
+    if (e != hipSuccess) return bfail(b, EWPT_ERR_HIP, "batch sync", e);
+    batch_commit(b, nsteps);
+    return EWPT_OK;
+}
+
+int ewpt_batch_step_timed(ewpt_batch *b, const ewpt_params *p, int nsteps, double *ms)
+{
+    if (!b || !ms || nsteps < 1) return EWPT_ERR_INVAL;
+    int rc = batch_check(b, p, nsteps);
+    if (rc) return rc;
+    // capture and upload outside the timed region
+    if (nsteps >= 2 && (rc = ensure_graph(b, any_beta(b, p)))) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return bfail(b, EWPT_ERR_HIP, "event");
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return bfail(b, EWPT_ERR_HIP, "event"); }
+    if ((rc = batch_upload(b, p)) == EWPT_OK) {
+        (void)hipEventRecord(e0, b->stream);
+        rc = batch_run(b, p, nsteps, true);
+        (void)hipEventRecord(e1, b->stream);
+    }
+    const hipError_t e = hipStreamSynchronize(b->stream);
+    float f = 0.f;
+    if (rc == EWPT_OK && e == hipSuccess) (void)hipEventElapsedTime(&f, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (e != hipSuccess) return bfail(b, EWPT_ERR_HIP, "batch sync", e);
+    batch_commit(b, nsteps);
+    *ms = f;
+    return EWPT_OK;
+}
+
+int ewpt_batch_tune(ewpt_batch *b, const ewpt_params *p, int nsteps, int *best_variant, double *best_ms)
+{
+    if (!b || !p || nsteps < 1) return EWPT_ERR_INVAL;
+    const std::string key = batch_key(b);
+    auto hit = tune_cache().find(key);
+    if (hit != tune_cache().end()) {
+        int rc = ewpt_batch_set_variant(b, hit->second);
+        if (rc) return rc;
+        if (best_variant) *best_variant = hit->second;
+        if (best_ms) *best_ms = -1.0;   // served from the cache
+        return EWPT_OK;
+    }
+    // dt = 0 leaves every state bitwise unchanged (y' = y + 0), so timing does not
+    // disturb the lattices
+    std::vector<ewpt_params> q(p, p + b->n);
+    for (ewpt_params &x : q) x.dt = 0.0;
+    int best = -1;
+    double tbest = 1e300;
+    for (int v : batch_candidates(b->precision == EWPT_MIXED)) {
+        if (!batch_fits_all(b, v)) continue;
+        int rc = ewpt_batch_set_variant(b, v);
+        if (rc == EWPT_OK) rc = ewpt_batch_step(b, q.data(), 2);   // warm-up: capture, first launches
+        double ms = 0.0;
+        if (rc == EWPT_OK) rc = ewpt_batch_step_timed(b, q.data(), nsteps, &ms);
+        if (rc) return rc;
+        if (ms < tbest) { tbest = ms; best = v; }
+    }
+    if (best < 0) return bfail(b, EWPT_ERR_UNSUPPORTED, "no batch variant tiles every lattice");
+    tune_cache()[key] = best;
+    int rc = ewpt_batch_set_variant(b, best);
+    if (rc) return rc;
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = tbest / nsteps;
     return EWPT_OK;
 }
 

@@ -227,10 +227,14 @@ __host__ __device__ constexpr int pf_of_stage(int PF, int S)
 // FP64 (256 VGPRs, also for 256-thread bricks at two workgroups per CU) and
 // four for FP32 stages 1-3 (two 512-thread workgroups per CU); the FP32
 // stage 4 parks more in LDS and runs one workgroup per CU.
-template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T = double>
-__global__ void __launch_bounds__(TX *TY *TZ)
-__attribute__((amdgpu_waves_per_eu(std::is_same_v<T, float> && S < 4 ? 4 : 2)))
-stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
+//
+// brick_body computes brick `bid` (enumeration index, brick_coords) of the
+// lattice described by b and g, using the dynamic LDS at lds_raw; t is the
+// thread's index in the workgroup.  It is called by the single-lattice
+// kernel stage_brick and by the batched kernels stage_brick_batch.
+template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T>
+__device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, int bid,
+                                           unsigned char *lds_raw, int t)
 {
     // T = float is the mixed-precision path: y is stored as an FP32 pair
     // (b.y = high word, b.ylo = compensation word), stages read the high
@@ -240,24 +244,14 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
     using B = BrickShape<TX, TY, TZ>;
     using R = T;
     // dynamic LDS: two staging buffers, then (stage 4) the 20 partial sums n
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     R *lds_base = reinterpret_cast<R *>(lds_raw);
     R *lds[2] = {lds_base, lds_base + B::BUF};
     R *nstash = lds_base + 2 * B::BUF;
 
     // ---- brick of this workgroup
-    int bid = blockIdx.x;
-    const int nbrick = g.nbx * g.nby * g.nbz;
-    if (g.xcd_map) {
-        // consecutive ids go to different XCDs; give XCD x the contiguous range
-        // [x*nbrick/8, (x+1)*nbrick/8) so its resident workgroups are neighbours
-        const int per = nbrick >> 3;
-        bid = (bid & 7) * per + (bid >> 3);
-    }
     int bx, by, bz;
     brick_coords(g, bid, bx, by, bz);
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
-    const int t = threadIdx.x;
     const int tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
     const long nxl = g.nx, nxy = (long)g.nx * g.ny;
     const unsigned qb = (unsigned)((x0 + tx) + nxl * (y0 + ty) + nxy * (z0 + tz)) * (unsigned)sizeof(T);
@@ -644,6 +638,22 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 #pragma unroll
         for (int m = 0; m < NK; m++) sto(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
     }
+}
+
+template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T = double>
+__global__ void __launch_bounds__(TX *TY *TZ)
+__attribute__((amdgpu_waves_per_eu(std::is_same_v<T, float> && S < 4 ? 4 : 2)))
+stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    int bid = blockIdx.x;
+    if (g.xcd_map) {
+        // consecutive ids go to different XCDs; give XCD x the contiguous range
+        // [x*nbrick/8, (x+1)*nbrick/8) so its resident workgroups are neighbours
+        const int per = (g.nbx * g.nby * g.nbz) >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    brick_body<TX, TY, TZ, PF, BETA, S, T>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x);
 }
 
 template <int TX, int TY, int TZ>

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/ewpt.h"
 #include "ewpt_point.h"
 #include "ewpt_stage.h"
@@ -71,6 +73,26 @@ hipError_t launch_add_bubbles(double *y, const ewpt_lattice &L, const double *db
 // returns 0, or -1 with *what naming the failed call; work holds 3*N^3 complex doubles
 int thermal_state(double *y, const ewpt_lattice &L, double T, double meff_sq, unsigned long long seed, int profile,
                   void *work, hipStream_t st, const char **what);
+
+// ewpt_batch.hip -- one persistent launch per stage over the bricks of many
+// lattices.  BatchLat describes one lattice: its stage buffers for parity 0
+// (the context's current state) and parity 1 (state and update swapped).
+template <typename T>
+struct BatchLat {
+    ewpt_lattice L;
+    Phys P;
+    double dt;
+    StageBufs<T> b[2][4];
+};
+struct BatchPlan;
+bool batch_variant_fits(int variant, bool f32, const ewpt_lattice &L);
+BatchPlan *batch_plan_create(int variant, bool f32, const std::vector<ewpt_lattice> &lats, int device,
+                             const char **what);
+void batch_plan_destroy(BatchPlan *p);
+// upload the descriptors of lats (BatchLat<float> or BatchLat<double>, one per lattice); 0 or -1
+int batch_plan_set(BatchPlan *p, const void *lats, hipStream_t st);
+hipError_t batch_plan_launch(const BatchPlan *p, int parity, int stage, bool beta, hipStream_t st);
+int batch_plan_tiles(const BatchPlan *p);
 
 inline Phys phys_of(const ewpt_params &p)
 {

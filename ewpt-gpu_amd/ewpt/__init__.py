@@ -4,9 +4,10 @@ Mirrors the driver surface of Teerthal/EWPT-GPU (gpu-kernels-rk4/main_rk4.jl
 run(), parameters.jl) over the C ABI of libewpt (include/ewpt.h).
 """
 from ._lib import (EWPT_FP64, EWPT_MIXED, EwptError, EwptLattice, EwptParams, NCH, NK, load)
+from .batch import Batch
 from .ic import bubble_table, gen_phi
 from .lattice import Lattice, as_params
 from .parameters import Parameters, J_ext, damp
 
 __all__ = ["EWPT_FP64", "EWPT_MIXED", "EwptError", "EwptLattice", "EwptParams", "NCH", "NK", "load",
-           "Lattice", "as_params", "bubble_table", "gen_phi", "Parameters", "J_ext", "damp"]
+           "Batch", "Lattice", "as_params", "bubble_table", "gen_phi", "Parameters", "J_ext", "damp"]

@@ -33,6 +33,8 @@ EXPORTS = (
     "ewpt_set_variant", "ewpt_get_variant", "ewpt_device_bytes", "ewpt_strerror",
     "ewpt_last_error", "ewpt_version", "ewpt_init_modes", "ewpt_init_bubbles", "ewpt_init_thermal",
     "ewpt_em_energies", "ewpt_em_fields", "ewpt_spectrum", "ewpt_y_slice",
+    "ewpt_batch_create", "ewpt_batch_destroy", "ewpt_batch_last_error", "ewpt_batch_set_variant",
+    "ewpt_batch_get_variant", "ewpt_batch_step", "ewpt_batch_step_timed", "ewpt_batch_tune",
 )
 
 
@@ -89,6 +91,13 @@ def load() -> ctypes.CDLL:
         "ewpt_em_energies": [vp, pp, ctypes.c_double, dp],
         "ewpt_em_fields": [vp, pp, ctypes.c_double, dp],
         "ewpt_y_slice": [vp, pp, ctypes.c_double, ip, dp],
+        "ewpt_batch_create": [ctypes.POINTER(vp), ctypes.POINTER(vp), ip],
+        "ewpt_batch_destroy": [vp],
+        "ewpt_batch_set_variant": [vp, ip],
+        "ewpt_batch_get_variant": [vp, ctypes.POINTER(ctypes.c_int)],
+        "ewpt_batch_step": [vp, pp, ip],
+        "ewpt_batch_step_timed": [vp, pp, ip, dp],
+        "ewpt_batch_tune": [vp, pp, ip, ctypes.POINTER(ctypes.c_int), dp],
         "ewpt_spectrum": [vp, pp, ctypes.c_double, ip, ctypes.POINTER(ctypes.c_int), ip, dp, dp,
                           ctypes.POINTER(ctypes.c_int)],
     }
@@ -98,6 +107,8 @@ def load() -> ctypes.CDLL:
         f.restype = ctypes.c_int
     L.ewpt_strerror.argtypes = [ip]
     L.ewpt_strerror.restype = ctypes.c_char_p
+    L.ewpt_batch_last_error.argtypes = [vp]
+    L.ewpt_batch_last_error.restype = ctypes.c_char_p
     L.ewpt_last_error.argtypes = [vp]
     L.ewpt_last_error.restype = ctypes.c_char_p
     L.ewpt_version.argtypes = []

@@ -213,6 +213,37 @@ const char *ewpt_last_error(ewpt_ctx *ctx);
 /* Library version string. */
 const char *ewpt_version(void);
 
+/* ---- batches: many independent lattices stepped together.
+ * No counterpart in the reference, which runs one lattice per process
+ * (the configuration with many seeded realisations of uneven sizes).
+ * A batch refers to existing contexts (same device and precision; each
+ * keeps its own state, ICs, energies, spectra).  One step of the batch is
+ * one persistent launch per RK4 stage over the bricks of every lattice; two
+ * steps are captured once in a hipGraph and replayed.  The result for each
+ * lattice is bitwise that of ewpt_step on it alone with the same stage
+ * variant.  The contexts must not be used by other calls while a batch call
+ * runs; destroying a context invalidates the batch. */
+typedef struct ewpt_batch ewpt_batch;
+int ewpt_batch_create(ewpt_batch **out, ewpt_ctx *const *ctxs, int n);
+int ewpt_batch_destroy(ewpt_batch *b);
+const char *ewpt_batch_last_error(ewpt_batch *b);
+/* Brick variant for every lattice (as ewpt_set_variant: 10/11 (+100*order)
+ * for both precisions, also 2/3 for FP64), plus 1000 for the persistent
+ * launch form (a resident grid striding through the bricks instead of one
+ * workgroup per brick); 0 = first that tiles all. */
+int ewpt_batch_set_variant(ewpt_batch *b, int variant);
+int ewpt_batch_get_variant(ewpt_batch *b, int *variant);
+/* nsteps RK4 steps of every lattice; p holds one ewpt_params per lattice. */
+int ewpt_batch_step(ewpt_batch *b, const ewpt_params *p, int nsteps);
+/* Same, and *ms = device time of the nsteps steps (HIP events). */
+int ewpt_batch_step_timed(ewpt_batch *b, const ewpt_params *p, int nsteps, double *ms);
+/* Autotuner: times every candidate variant that tiles all lattices over
+ * nsteps steps with dt set to 0 (which leaves every state bitwise
+ * unchanged), selects the fastest, and caches the choice for this set of
+ * lattice shapes for the life of the process.  *best_ms = ms per step of the
+ * winner (-1 when the choice came from the cache). */
+int ewpt_batch_tune(ewpt_batch *b, const ewpt_params *p, int nsteps, int *best_variant, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif
