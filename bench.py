@@ -124,6 +124,116 @@ def batch_bench(a) -> int:
     return 0
 
 
+def corrector_bench(a) -> int:
+    """Learned coarse-step corrector: train on 128^3 thermal trajectories,
+    evaluate at 256^3 over 2000 fine-step equivalents against plain RK4 at
+    dt, 2 dt and 4 dt (errors against the dt run)."""
+    import numpy as np
+
+    import ewpt
+
+    dx, factor = 0.25, 4
+    n_train, n_eval, fine_steps = a.corr_train_n, a.corr_eval_n, a.corr_fine_steps
+    par = ewpt.Parameters(N=n_train, dx=dx, gy=0.35)
+    p = par.step_params()
+    rng = np.random.default_rng(7)
+    t0 = time.perf_counter()
+    src, co, fi = (ewpt.Lattice(n_train, dx) for _ in range(3))
+    net = ewpt.Corrector(seed=3)
+    net.input_mode = 1            # lattice Laplacian of the coarse state
+    try:
+        cs, rs = [], []
+        for k in range(6):
+            src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 900 + k, "config")
+            src.step(p, int(k * fine_steps / 12))
+            ewpt.make_pair(src, co, fi, p, factor)
+            c, f = co.get_state(), fi.get_state()
+            cs.append(net.features(c).reshape(36, -1).std(1))
+            rs.append((f - c).reshape(36, -1).std(1))
+        net.set_norm(np.maximum(np.mean(cs, 0), 1e-12), np.maximum(np.mean(rs, 0), 1e-30))
+        losses = []
+        for it in range(a.corr_iters):
+            if it % 100 == 0:   # a new trajectory: fresh seed and temperature, random start time
+                src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 1000 + it, "config")
+                src.step(p, int(rng.integers(0, fine_steps // 2)))
+            ewpt.make_pair(src, co, fi, p, factor)
+            losses.append(net.train_step(co, fi, a.corr_lr))
+            src.step(p, int(rng.integers(1, 5)))
+            if (it + 1) % 250 == 0:
+                print(f"bench: corrector training {it + 1}/{a.corr_iters}, relative loss "
+                      f"{np.mean(losses[-100:]):.3f} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+        train_s = time.perf_counter() - t0
+    finally:
+        for x in (src, co, fi):
+            x.close()
+    print(f"bench: corrector trained {a.corr_iters} iterations on {n_train}^3 in {train_s:.1f} s, "
+          f"relative loss {np.mean(losses[:50]):.3f} -> {np.mean(losses[-200:]):.3f}", file=sys.stderr, flush=True)
+
+    pe = ewpt.Parameters(N=n_eval, dx=dx, gy=0.35)
+    p = pe.step_params()
+
+    def with_dt(m):
+        q = pe.step_params()
+        q.dt = m * p.dt
+        return q
+
+    base = ewpt.Lattice(n_eval, dx)
+    base.init_thermal(0.5, pe.meff_sq, 424242, "config")
+    e0 = base.energies(p)
+    runs = {}
+    plan = [("rk4_dt", 1), ("rk4_2dt", 2), ("rk4_4dt", 4), ("corrected_4dt", 4)]
+    for name, m in plan:
+        lat = ewpt.Lattice(n_eval, dx)
+        lat.copy_from(base)
+        nfb = 0
+        w0 = time.perf_counter()
+        if name.startswith("corrected"):
+            nfb = net.step(lat, p, factor=m, ncoarse=fine_steps // m, gauss_tol=a.corr_tol)
+        else:
+            lat.step(with_dt(m), fine_steps // m)
+        wall = time.perf_counter() - w0
+        runs[name] = (lat, wall, nfb)
+        print(f"bench: corrector eval {name}: {wall:.1f} s", file=sys.stderr, flush=True)
+    ref = runs["rk4_dt"][0].get_state()[:16]
+    refn = np.linalg.norm(ref)
+    tphys = fine_steps * p.dt
+    res = {}
+    for name, (lat, wall, nfb) in runs.items():
+        e = lat.energies(p)
+        g = lat.gauss(p)
+        err = 0.0 if name == "rk4_dt" else float(np.linalg.norm(lat.get_state()[:16] - ref) / refn)
+        res[name] = {"wall_s": wall, "physical_time_per_s": tphys / wall, "field_rel_l2_vs_dt": err,
+                     "energy_drift": float((sum(e[:7]) - sum(e0[:7])) / sum(e0[:7])),
+                     "gauss_l2": float(np.sqrt(np.sum(np.asarray(g[:4]) ** 2))), "fallbacks": nfb}
+        lat.close()
+    base.close()
+    net.close()
+    # plain RK4 at the corrected run's error: RK4 error scales as dt^4 between the 2 dt and 4 dt runs
+    e2, e4, ec = res["rk4_2dt"]["field_rel_l2_vs_dt"], res["rk4_4dt"]["field_rel_l2_vs_dt"], \
+        res["corrected_4dt"]["field_rel_l2_vs_dt"]
+    order = float(np.log(e4 / e2) / np.log(2.0)) if e2 > 0 and e4 > 0 else None
+    m_eq = 4.0 * (ec / e4) ** (1.0 / order) if order and ec > 0 else None
+    rk4_eq = res["rk4_dt"]["physical_time_per_s"] * m_eq if m_eq else None
+    out = {
+        "metric": f"physical time per second at {n_eval}^3 FP64: learned 4*dt corrector vs plain RK4",
+        "value": res["corrected_4dt"]["physical_time_per_s"], "unit": "physical time units / s", "n_gpus": 1,
+        "steps": fine_steps, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 state, bf16 network (fp32 accumulate)",
+        "data": "synthetic: seeded thermal ICs (training T in [0.3, 0.7], evaluation T = 0.5)",
+        "config": {"workload": f"{fine_steps} fine-step equivalents ({fine_steps // factor} corrected 4*dt steps) at "
+                               f"{n_eval}^3, network trained on {n_train}^3",
+                   "train_iters": a.corr_iters, "train_s": train_s, "lr": a.corr_lr, "gauss_tol": a.corr_tol,
+                   "input": "7-point lattice Laplacian of the coarse state"},
+        "error_reduction_vs_rk4_4dt": ec / e4 if e4 > 0 else None,
+        "rk4_observed_order": order,
+        "rk4_step_multiple_at_same_error": m_eq,
+        "rk4_physical_time_per_s_at_same_error": rk4_eq,
+        "runs": res,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,12 +249,27 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", action="store_true",
                     help="the batched configuration instead: 48 realisations (16 each at 64^3, 96^3, 128^3), "
                          "half thermal, half bubbles, mixed precision, autotuned, graph-captured steps")
+    ap.add_argument("--corrector", action="store_true",
+                    help="the learned-corrector configuration instead: train on 128^3, evaluate at 256^3 "
+                         "over 2000 fine-step equivalents against plain RK4")
+    ap.add_argument("--corr-train-n", type=int, default=128)
+    ap.add_argument("--corr-eval-n", type=int, default=256)
+    ap.add_argument("--corr-fine-steps", type=int, default=2000)
+    ap.add_argument("--corr-iters", type=int, default=3000)
+    ap.add_argument("--corr-lr", type=float, default=3e-4)
+    ap.add_argument("--corr-tol", type=float, default=1e-4)
     ap.add_argument("--stream-copy", action="store_true",
                     help="only time a plain device copy (HBM ceiling / counter calibration)")
     a = ap.parse_args(argv)
 
     if a.stream_copy:
         return stream_copy()
+
+    if a.corrector:
+        if a.gpus != 1:
+            print("bench.py: the corrector configuration runs on one GPU", file=sys.stderr)
+            return 2
+        return corrector_bench(a)
 
     if a.batch:
         if a.gpus != 1:

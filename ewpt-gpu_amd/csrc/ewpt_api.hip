@@ -288,6 +288,28 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host)
     return EWPT_OK;
 }
 
+int ewpt_copy_state(ewpt_ctx *dst, ewpt_ctx *src)
+{
+    if (!dst || !src || dst == src) return EWPT_ERR_INVAL;
+    if (dst->device != src->device || dst->precision != src->precision || dst->lat.nx != src->lat.nx ||
+        dst->lat.ny != src->lat.ny || dst->lat.nz != src->lat.nz)
+        return fail(dst, EWPT_ERR_INVAL, "copy_state: contexts differ in device, precision or lattice");
+    if (!src->has_state) return fail(dst, EWPT_ERR_STATE, "copy_state: source has no state");
+    int rc = set_device(src);
+    if (rc) return rc;
+    const size_t n = NCH * (size_t)src->n3;
+    if (mixed(src)) {
+        HIPCHK(dst, hipMemcpyAsync(dst->fhi, src->fhi, n * sizeof(float), hipMemcpyDeviceToDevice, src->stream), "copy hi");
+        HIPCHK(dst, hipMemcpyAsync(dst->flo, src->flo, n * sizeof(float), hipMemcpyDeviceToDevice, src->stream), "copy lo");
+        dst->view_valid = false;
+    } else {
+        HIPCHK(dst, hipMemcpyAsync(dst->y, src->y, n * sizeof(double), hipMemcpyDeviceToDevice, src->stream), "copy y");
+    }
+    HIPCHK(dst, hipStreamSynchronize(src->stream), "copy sync");
+    dst->has_state = true;
+    return EWPT_OK;
+}
+
 int ewpt_init_modes(ewpt_ctx *ctx, unsigned long long seed, int nmodes, int kmax, double amp, double vev,
                     double mom_scale, double *modes_out)
 {
@@ -839,6 +861,226 @@ std::map<std::string, int> &tune_cache()
 }  // namespace
 
 extern "C" {
+
+// ---------------------------------------------------------------- learned corrector
+
+}  // extern "C"
+
+struct ewpt_net {
+    Net *net = nullptr;
+    int device = 0;
+    double *save = nullptr;   // state before a corrected coarse step (fallback)
+    long save_n = 0;
+    std::string err;
+};
+
+namespace {
+
+int nfail(ewpt_net *n, int code, const char *what, hipError_t e = hipSuccess)
+{
+    char buf[512];
+    if (e != hipSuccess) std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else std::snprintf(buf, sizeof buf, "%s", what);
+    n->err = buf;
+    return code;
+}
+
+// one reduction pass: G = sqrt(sum of the squared Gauss-law norms G^1..3,
+// G^Y) and ||E|| = sqrt(sum (E^a)^2 + (E^Y)^2) dx^3 (= sqrt(2 (EEW + EEY)))
+int gauss_and_enorm(ewpt_ctx *c, const ewpt_params *p, double *G, double *En)
+{
+    int rc = set_device(c);
+    if (rc) return rc;
+    if ((rc = ensure_view(c))) return rc;
+    HIPCHK(c, launch_gauss(c->y, c->lat, phys_of(*p), c->part, c->res, c->stream), "gauss launch");
+    double r[10];
+    HIPCHK(c, hipMemcpyAsync(r, c->res, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "gauss copy");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "gauss sync");
+    const double vol = c->lat.dx * c->lat.dx * c->lat.dx;
+    *G = std::sqrt((r[0] + r[1] + r[2] + r[3]) * vol);
+    *En = std::sqrt((r[8] + r[9]) * vol);
+    if (!std::isfinite(*G) || !std::isfinite(*En)) return EWPT_ERR_NONFINITE;
+    return EWPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ewpt_net_create(ewpt_net **out, int device, unsigned long long seed)
+{
+    if (!out || device < 0) return EWPT_ERR_INVAL;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return EWPT_ERR_HIP;
+    ewpt_net *n = new (std::nothrow) ewpt_net();
+    if (!n) return EWPT_ERR_OOM;
+    const char *what = "";
+    n->device = device;
+    n->net = net_create(device, seed, &what);
+    if (!n->net) {
+        delete n;
+        return EWPT_ERR_HIP;
+    }
+    *out = n;
+    return EWPT_OK;
+}
+
+int ewpt_net_destroy(ewpt_net *n)
+{
+    if (!n) return EWPT_ERR_INVAL;
+    (void)hipSetDevice(n->device);
+    net_destroy(n->net);
+    if (n->save) (void)hipFree(n->save);
+    delete n;
+    return EWPT_OK;
+}
+
+const char *ewpt_net_last_error(ewpt_net *n) { return n ? n->err.c_str() : ""; }
+
+long ewpt_net_param_count(void) { return net_param_count(); }
+
+int ewpt_net_set_params(ewpt_net *n, const float *params)
+{
+    if (!n || !params) return EWPT_ERR_INVAL;
+    for (long q = 0; q < net_param_count(); q++)
+        if (!std::isfinite(params[q])) return nfail(n, EWPT_ERR_INVAL, "net: non-finite parameter");
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const hipError_t e = net_set_params(n->net, params);
+    return e == hipSuccess ? EWPT_OK : nfail(n, EWPT_ERR_HIP, "net: set parameters", e);
+}
+
+int ewpt_net_get_params(ewpt_net *n, float *params)
+{
+    if (!n || !params) return EWPT_ERR_INVAL;
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const hipError_t e = net_get(n->net, 0, params);
+    return e == hipSuccess ? EWPT_OK : nfail(n, EWPT_ERR_HIP, "net: get parameters", e);
+}
+
+int ewpt_net_get_grads(ewpt_net *n, float *grads)
+{
+    if (!n || !grads) return EWPT_ERR_INVAL;
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const hipError_t e = net_get(n->net, 1, grads);
+    return e == hipSuccess ? EWPT_OK : nfail(n, EWPT_ERR_HIP, "net: get gradients", e);
+}
+
+int ewpt_net_set_norm(ewpt_net *n, const float in_scale[36], const float out_scale[36])
+{
+    if (!n || !in_scale || !out_scale) return EWPT_ERR_INVAL;
+    float inv[NCH];
+    for (int c = 0; c < NCH; c++) {
+        if (!std::isfinite(in_scale[c]) || !(in_scale[c] > 0.f) || !std::isfinite(out_scale[c]))
+            return nfail(n, EWPT_ERR_INVAL, "net: scales must be finite, input scales positive");
+        inv[c] = 1.0f / in_scale[c];
+    }
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const hipError_t e = net_set_norm(n->net, inv, out_scale);
+    return e == hipSuccess ? EWPT_OK : nfail(n, EWPT_ERR_HIP, "net: set scales", e);
+}
+
+static int net_ctx_ok(ewpt_net *n, ewpt_ctx *c)
+{
+    if (c->device != n->device) return nfail(n, EWPT_ERR_INVAL, "net: context on another device");
+    if (c->precision != EWPT_FP64) return nfail(n, EWPT_ERR_UNSUPPORTED, "net: FP64 contexts only");
+    if (!net_lattice_ok(c->lat)) return nfail(n, EWPT_ERR_UNSUPPORTED, "net: lattice must tile 8x4x4");
+    if (!c->has_state) return nfail(n, EWPT_ERR_STATE, "net: context has no state");
+    return EWPT_OK;
+}
+
+int ewpt_net_set_input(ewpt_net *n, int mode)
+{
+    if (!n || (mode != 0 && mode != 1)) return EWPT_ERR_INVAL;
+    net_set_input_mode(n->net, mode);
+    return EWPT_OK;
+}
+
+int ewpt_net_apply(ewpt_net *n, ewpt_ctx *ctx)
+{
+    if (!n || !ctx) return EWPT_ERR_INVAL;
+    int rc = net_ctx_ok(n, ctx);
+    if (rc) return rc;
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const hipError_t e = net_apply(n->net, ctx->y, ctx->lat);
+    return e == hipSuccess ? EWPT_OK : nfail(n, EWPT_ERR_HIP, "net: apply", e);
+}
+
+int ewpt_net_train_step(ewpt_net *n, ewpt_ctx *coarse, ewpt_ctx *fine, double lr, double *loss)
+{
+    if (!n || !coarse || !fine || !loss || !std::isfinite(lr) || lr < 0.0) return EWPT_ERR_INVAL;
+    int rc = net_ctx_ok(n, coarse);
+    if (rc || (rc = net_ctx_ok(n, fine))) return rc;
+    if (coarse->lat.nx != fine->lat.nx || coarse->lat.ny != fine->lat.ny || coarse->lat.nz != fine->lat.nz)
+        return nfail(n, EWPT_ERR_INVAL, "net: pair of different lattices");
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const hipError_t e = net_train(n->net, coarse->y, fine->y, coarse->lat, lr, loss);
+    return e == hipSuccess ? EWPT_OK : nfail(n, EWPT_ERR_HIP, "net: train step", e);
+}
+
+int ewpt_step_corrected(ewpt_ctx *ctx, ewpt_net *n, const ewpt_params *p, int factor, int ncoarse,
+                        double gauss_tol, int *nfallback)
+{
+    if (!ctx || !n || !params_ok(p) || factor < 1 || ncoarse < 0 || std::isnan(gauss_tol)) return EWPT_ERR_INVAL;
+    int rc = net_ctx_ok(n, ctx);
+    if (rc) return rc;
+    if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
+    const long need = NCH * ctx->n3;
+    if (n->save_n != need) {
+        if (n->save) (void)hipFree(n->save);
+        n->save = nullptr;
+        n->save_n = 0;
+        if (hipMalloc(reinterpret_cast<void **>(&n->save), need * sizeof(double)) != hipSuccess)
+            return nfail(n, EWPT_ERR_OOM, "net: alloc fallback copy");
+        n->save_n = need;
+    }
+    ewpt_params pc = *p;
+    pc.dt = p->dt * factor;
+    int fb = 0;
+    const bool check = !std::isinf(gauss_tol) || gauss_tol < 0;   // +inf: never fall back, no checks
+    double g0 = 0.0, e0 = 0.0;
+    if (check && (rc = gauss_and_enorm(ctx, p, &g0, &e0))) return rc;
+    for (int it = 0; it < ncoarse; it++) {
+        double g1 = 0.0, e1 = 0.0;
+        if (check)
+            HIPCHK(ctx, hipMemcpyAsync(n->save, ctx->y, need * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+                   "save state");
+        if ((rc = ewpt_step(ctx, &pc, 1))) return rc;
+        if ((rc = ewpt_net_apply(n, ctx))) return rc;
+        if (!check) continue;
+        if ((rc = gauss_and_enorm(ctx, p, &g1, &e1)) && rc != EWPT_ERR_NONFINITE) return rc;
+        const double drift = (g1 - g0) / std::max(e1, 1e-12);
+        if (rc == EWPT_ERR_NONFINITE || !(drift <= gauss_tol)) {
+            // the constraint error grew past the tolerance: redo with fine steps
+            HIPCHK(ctx, hipMemcpyAsync(ctx->y, n->save, need * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+                   "restore state");
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "restore sync");
+            if ((rc = ewpt_step(ctx, p, factor))) return rc;
+            if ((rc = gauss_and_enorm(ctx, p, &g1, &e1))) return rc;
+            fb++;
+        }
+        g0 = g1;   // the residual after this step is the next step's reference
+    }
+    if (nfallback) *nfallback = fb;
+    return EWPT_OK;
+}
+
+int ewpt_net_conv_test(int nx, int ny, int nz, int mode, const unsigned short *X, const unsigned short *W,
+                       const float *bias, const float *zprev, float *Z, unsigned short *A)
+{
+    if (!X || !W || mode < 0 || mode > 2 || (mode < 2 && !bias) || (mode == 2 && (!zprev || !A)) ||
+        (mode < 2 && !Z) || (mode == 0 && !A))
+        return EWPT_ERR_INVAL;
+    if (!net_lattice_ok(ewpt_lattice{nx, ny, nz, 1.0}) || nx < 8 || ny < 4 || nz < 4) return EWPT_ERR_INVAL;
+    return net_conv_host(nx, ny, nz, mode, X, W, bias, zprev, Z, A) == hipSuccess ? EWPT_OK : EWPT_ERR_HIP;
+}
+
+int ewpt_net_wgrad_test(int nx, int ny, int nz, const unsigned short *dZ, const unsigned short *X, float *dW,
+                        float *db)
+{
+    if (!dZ || !X || !dW || !db) return EWPT_ERR_INVAL;
+    if (!net_lattice_ok(ewpt_lattice{nx, ny, nz, 1.0}) || nx < 8 || ny < 4 || nz < 4) return EWPT_ERR_INVAL;
+    return net_wgrad_host(nx, ny, nz, dZ, X, dW, db) == hipSuccess ? EWPT_OK : EWPT_ERR_HIP;
+}
 
 int ewpt_batch_create(ewpt_batch **out, ewpt_ctx *const *ctxs, int n)
 {

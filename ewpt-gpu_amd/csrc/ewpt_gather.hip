@@ -270,13 +270,14 @@ __global__ void __launch_bounds__(256) y_slice_kernel(const double *y, long n3, 
     out[6 * np + q] = 0.0;
 }
 
-// Gauss residuals (squared), slots 0..7 as documented in ewpt.h.
+// Gauss residuals (squared), slots 0..7 as documented in ewpt.h; slots 8, 9:
+// sum |E^a|^2, sum |E^Y|^2.
 __global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3, Geo g, Phys P, double *part)
 {
     const int i = blockIdx.x * BX + threadIdx.x;
     const int j = blockIdx.y * BY + threadIdx.y;
     const int k = blockIdx.z;
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (i < g.nx && j < g.ny) {
         StageBufs<double> b{};
         b.y = y;
@@ -302,8 +303,12 @@ __global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3
         v[3] = rY * rY;
         const double cY = s.c[CH_SIG] - (s.dY[0][0] + s.dY[1][1] + s.dY[2][2]);
         v[7] = cY * cY;
+        // |E^a|^2 and |E^Y|^2, the scale the residuals are judged against
+#pragma unroll
+        for (int q = 0; q < 9; q++) v[8] += s.c[CH_E + q] * s.c[CH_E + q];
+        v[9] = s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2];
     }
-    block_reduce<8>(v, 0u, part);
+    block_reduce<10>(v, 0u, part);
 }
 
 // Electromagnetic fields (em_point): the channel groups selected by mask
@@ -434,7 +439,7 @@ hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, d
     const Geo g = make_geo(L);
     const long n3 = (long)L.nx * L.ny * L.nz;
     gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
-    finish_reduce<<<8, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
     return hipGetLastError();
 }
 

@@ -94,6 +94,23 @@ int batch_plan_set(BatchPlan *p, const void *lats, hipStream_t st);
 hipError_t batch_plan_launch(const BatchPlan *p, int parity, int stage, bool beta, hipStream_t st);
 int batch_plan_tiles(const BatchPlan *p);
 
+// ewpt_net.hip -- learned corrector (3x3x3 conv net on the matrix cores)
+struct Net;
+bool net_lattice_ok(const ewpt_lattice &L);
+Net *net_create(int device, unsigned long long seed, const char **what);
+void net_destroy(Net *n);
+long net_param_count();
+hipError_t net_set_params(Net *n, const float *host);
+hipError_t net_get(Net *n, int which, float *host);   // 0 = parameters, 1 = last gradients
+hipError_t net_set_norm(Net *n, const float *in_inv, const float *out_scale);
+void net_set_input_mode(Net *n, int mode);   // 0 state, 1 lattice Laplacian of the state
+hipError_t net_apply(Net *n, double *y, const ewpt_lattice &L);
+hipError_t net_train(Net *n, const double *yc, const double *yf, const ewpt_lattice &L, double lr, double *loss);
+hipError_t net_conv_host(int nx, int ny, int nz, int epi, const unsigned short *X, const unsigned short *W,
+                         const float *bias, const float *zprev, float *Zout, unsigned short *Aout);
+hipError_t net_wgrad_host(int nx, int ny, int nz, const unsigned short *dZ, const unsigned short *X, float *dW,
+                          float *db);
+
 inline Phys phys_of(const ewpt_params &p)
 {
     return Phys{p.gw, p.gy, p.gp2, p.beta_w, p.beta_y, p.vev, p.lambda, p.jex, p.gamma};

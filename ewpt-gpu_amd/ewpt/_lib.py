@@ -35,6 +35,10 @@ EXPORTS = (
     "ewpt_em_energies", "ewpt_em_fields", "ewpt_spectrum", "ewpt_y_slice",
     "ewpt_batch_create", "ewpt_batch_destroy", "ewpt_batch_last_error", "ewpt_batch_set_variant",
     "ewpt_batch_get_variant", "ewpt_batch_step", "ewpt_batch_step_timed", "ewpt_batch_tune",
+    "ewpt_copy_state", "ewpt_net_create", "ewpt_net_destroy", "ewpt_net_last_error", "ewpt_net_param_count",
+    "ewpt_net_set_params", "ewpt_net_get_params", "ewpt_net_get_grads", "ewpt_net_set_norm", "ewpt_net_apply",
+    "ewpt_net_train_step", "ewpt_step_corrected", "ewpt_net_conv_test", "ewpt_net_wgrad_test",
+    "ewpt_net_set_input",
 )
 
 
@@ -68,6 +72,7 @@ def load() -> ctypes.CDLL:
         raise RuntimeError(f"libewpt.so not built at {LIB_PATH}; run __graft_entry__.build()")
     L = ctypes.CDLL(LIB_PATH)
     vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+    fp, hp = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_ushort)
     lp = ctypes.POINTER(EwptLattice)
     pp = ctypes.POINTER(EwptParams)
     sig = {
@@ -98,6 +103,19 @@ def load() -> ctypes.CDLL:
         "ewpt_batch_step": [vp, pp, ip],
         "ewpt_batch_step_timed": [vp, pp, ip, dp],
         "ewpt_batch_tune": [vp, pp, ip, ctypes.POINTER(ctypes.c_int), dp],
+        "ewpt_copy_state": [vp, vp],
+        "ewpt_net_create": [ctypes.POINTER(vp), ip, ctypes.c_ulonglong],
+        "ewpt_net_destroy": [vp],
+        "ewpt_net_set_params": [vp, fp],
+        "ewpt_net_get_params": [vp, fp],
+        "ewpt_net_get_grads": [vp, fp],
+        "ewpt_net_set_norm": [vp, fp, fp],
+        "ewpt_net_apply": [vp, vp],
+        "ewpt_net_set_input": [vp, ip],
+        "ewpt_net_train_step": [vp, vp, vp, ctypes.c_double, dp],
+        "ewpt_step_corrected": [vp, vp, pp, ip, ip, ctypes.c_double, ctypes.POINTER(ctypes.c_int)],
+        "ewpt_net_conv_test": [ip, ip, ip, ip, hp, hp, fp, fp, fp, hp],
+        "ewpt_net_wgrad_test": [ip, ip, ip, hp, hp, fp, fp],
         "ewpt_spectrum": [vp, pp, ctypes.c_double, ip, ctypes.POINTER(ctypes.c_int), ip, dp, dp,
                           ctypes.POINTER(ctypes.c_int)],
     }
@@ -107,6 +125,10 @@ def load() -> ctypes.CDLL:
         f.restype = ctypes.c_int
     L.ewpt_strerror.argtypes = [ip]
     L.ewpt_strerror.restype = ctypes.c_char_p
+    L.ewpt_net_param_count.argtypes = []
+    L.ewpt_net_param_count.restype = ctypes.c_long
+    L.ewpt_net_last_error.argtypes = [vp]
+    L.ewpt_net_last_error.restype = ctypes.c_char_p
     L.ewpt_batch_last_error.argtypes = [vp]
     L.ewpt_batch_last_error.restype = ctypes.c_char_p
     L.ewpt_last_error.argtypes = [vp]

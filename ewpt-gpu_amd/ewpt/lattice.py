@@ -109,6 +109,10 @@ class Lattice:
         self._check(self._L.ewpt_init_thermal(self._h, T, meff_sq, ctypes.c_ulonglong(seed), profile_id(profile)),
                     "ewpt_init_thermal")
 
+    def copy_from(self, other: "Lattice"):
+        """Device-to-device copy of other's state (same lattice and precision)."""
+        self._check(self._L.ewpt_copy_state(self._h, other._h), "ewpt_copy_state")
+
     def get_state(self) -> np.ndarray:
         out = np.empty(self.shape)
         self._check(self._L.ewpt_get_state(self._h, _dptr(out)), "ewpt_get_state")

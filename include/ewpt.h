@@ -67,6 +67,10 @@ int ewpt_destroy(ewpt_ctx *ctx);
 int ewpt_set_state(ewpt_ctx *ctx, const double *host_state);
 int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
 
+/* Device-to-device copy of the state of src into dst (same device,
+ * precision and lattice). */
+int ewpt_copy_state(ewpt_ctx *dst, ewpt_ctx *src);
+
 /* Fill the context state on the device with a seeded smooth field: channel c
  * is the sum of nmodes cosines with wave numbers |k_x|,|k_y|,|k_z| <= kmax,
  * phases uniform in [0,2pi) and amplitudes uniform in [-amp,amp]/nmodes,
@@ -243,6 +247,55 @@ int ewpt_batch_step_timed(ewpt_batch *b, const ewpt_params *p, int nsteps, doubl
  * lattice shapes for the life of the process.  *best_ms = ms per step of the
  * winner (-1 when the choice came from the cache). */
 int ewpt_batch_tune(ewpt_batch *b, const ewpt_params *p, int nsteps, int *best_variant, double *best_ms);
+
+/* ---- learned coarse-step corrector.
+ * No counterpart in the reference; for callers that trade the fine RK4
+ * step for one step of factor*dt plus a learned correction.  A periodic
+ * 3x3x3 convolution network 36 -> 64 -> 64 -> 36 with SiLU, BF16 weights and
+ * activations, FP32 accumulation on the matrix cores, maps the coarse state
+ * y_c (channel c scaled by 1/in_scale[c]) to out; the corrected state is
+ * y_c + out_scale[c] * out[c].  Parameters: FP32, ewpt_net_param_count()
+ * values, layers 1..3 each [27 taps][64 out][64 in] weights then 64 biases
+ * (tap = (dz+1)*9 + (dy+1)*3 + (dx+1); in/out channels >= 36 of the first
+ * and last layer are padding).  Lattices must tile 8x4x4; FP64 contexts. */
+typedef struct ewpt_net ewpt_net;
+int ewpt_net_create(ewpt_net **out, int device, unsigned long long seed);
+int ewpt_net_destroy(ewpt_net *n);
+const char *ewpt_net_last_error(ewpt_net *n);
+long ewpt_net_param_count(void);
+int ewpt_net_set_params(ewpt_net *n, const float *params);
+int ewpt_net_get_params(ewpt_net *n, float *params);
+/* gradients of the last ewpt_net_train_step, same layout as the parameters */
+int ewpt_net_get_grads(ewpt_net *n, float *grads);
+int ewpt_net_set_norm(ewpt_net *n, const float in_scale[36], const float out_scale[36]);
+/* Network input: mode 0 = the coarse state, mode 1 = its 7-point lattice
+ * Laplacian (sum of the 6 neighbours - 6 x centre, FP64 before the bf16
+ * rounding); in_scale then scales the Laplacian. */
+int ewpt_net_set_input(ewpt_net *n, int mode);
+/* y <- y + out_scale * net(y / in_scale) on the context state */
+int ewpt_net_apply(ewpt_net *n, ewpt_ctx *ctx);
+/* One training step on the pair (coarse state, fine state): relative loss
+ * sum (out - T)^2 / sum T^2 over sites and the 36 channels,
+ * T = (y_fine - y_coarse)/out_scale (every pair weighs the same),
+ * gradients by the backward kernels, then Adam (beta 0.9, 0.999) with
+ * learning rate lr; lr = 0 computes loss and gradients only. */
+int ewpt_net_train_step(ewpt_net *n, ewpt_ctx *coarse, ewpt_ctx *fine, double lr, double *loss);
+/* ncoarse steps of factor*dt (p->dt is the fine step), each followed by the
+ * correction.  When the Gauss-law residual norm sqrt(sum G^2) grows over a
+ * coarse step by more than gauss_tol * ||E|| (||E||^2 = 2 (EEW + EEY)), the
+ * step is redone as factor fine steps; *nfallback counts those
+ * (gauss_tol = +inf never falls back and skips the residual evaluations). */
+int ewpt_step_corrected(ewpt_ctx *ctx, ewpt_net *n, const ewpt_params *p, int factor, int ncoarse,
+                        double gauss_tol, int *nfallback);
+/* Test hooks on host buffers: one convolution layer (mode 0: hidden layer,
+ * Z = conv + bias, A = bf16(silu(Z)); mode 1: last layer, Z only; mode 2:
+ * input gradient, A = bf16(conv(X, W) * silu'(zprev)) with W the mirrored,
+ * transposed weights), and one weight gradient (dW[tap][out][in],
+ * db[out]).  X, A, dZ: [site][64] bf16 bit patterns. */
+int ewpt_net_conv_test(int nx, int ny, int nz, int mode, const unsigned short *X, const unsigned short *W,
+                       const float *bias, const float *zprev, float *Z, unsigned short *A);
+int ewpt_net_wgrad_test(int nx, int ny, int nz, const unsigned short *dZ, const unsigned short *X,
+                        float *dW, float *db);
 
 #ifdef __cplusplus
 }
