@@ -140,17 +140,18 @@ int push_view(ewpt_ctx *c)
 }
 
 // variant actually launched: AUTO takes the first brick variant that tiles
-// the lattice from the measured ranking (512^3 FP64 on MI355X: 211, 210, 2),
-// else the gather kernel
+// the lattice from the measured ranking (512^3 on MI355X, FP64: 311 186.8,
+// 310 187.9, 210 190.0, 211 193.8 ms/step; mixed: 311 97.1, 211 103.0,
+// 310 133.9 ms/step), else the gather kernel
 int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
     if (mixed(c)) {
-        for (int v : {211, 210})
+        for (int v : {311, 211, 310, 210})
             if (brick_supported_f32(v, c->lat)) return v;
         return -1;
     }
-    for (int v : {211, 210, 2})
+    for (int v : {311, 310, 210, 211, 2})
         if (brick_supported(v, c->lat)) return v;
     return VARIANT_GATHER;
 }
@@ -680,8 +681,8 @@ void drop_plan(ewpt_batch *b)
 // candidate variants in preference order (measured single-lattice ranking)
 std::vector<int> batch_candidates(bool f32)
 {
-    std::vector<int> v = f32 ? std::vector<int>{210, 211, 110, 111, 10, 11}
-                             : std::vector<int>{211, 210, 111, 110, 11, 10, 203, 103, 3, 2};
+    std::vector<int> v = f32 ? std::vector<int>{311, 211, 411, 310, 210, 11, 10}
+                             : std::vector<int>{311, 310, 211, 210, 303, 203, 11, 10, 3, 2};
     const size_t n = v.size();
     for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms
     return v;

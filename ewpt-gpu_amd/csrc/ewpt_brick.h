@@ -83,8 +83,27 @@ __device__ __forceinline__ int wrapc(int v, int n) { return v < 0 ? v + n : (v >
 //   0: x fastest, then y, then z
 //   1: x fastest, then z, then y
 //   2: x fastest, then (y,z) in 4x4 tiles of brick rows (z fastest inside)
+//   3: cubes of 2x4x4 bricks (x,y,z), 32 consecutive indices per cube: the
+//      workgroups one XCD runs at the same time share most of their halos
+//   4: cubes of 4x4x4 bricks (64 per cube), for two workgroups per CU
+// Orders 3, 4 fall back to 0 where the brick counts do not divide.
 __device__ __forceinline__ void brick_coords(const BrickGeo &g, int i, int &bx, int &by, int &bz)
 {
+    if (g.order >= 3) {
+        const int cx = g.order == 3 ? 2 : 4, cy = 4, cz = 4, cn = cx * cy * cz;
+        if (g.nbx % cx == 0 && g.nby % cy == 0 && g.nbz % cz == 0) {
+            const int cube = i / cn, in = i % cn;
+            const int ncx = g.nbx / cx, ncy = g.nby / cy;
+            bx = (cube % ncx) * cx + in % cx;
+            by = ((cube / ncx) % ncy) * cy + (in / cx) % cy;
+            bz = (cube / (ncx * ncy)) * cz + in / (cx * cy);
+            return;
+        }
+        bx = i % g.nbx;
+        by = (i / g.nbx) % g.nby;
+        bz = i / (g.nbx * g.nby);
+        return;
+    }
     bx = i % g.nbx;
     const int r = i / g.nbx;   // brick-row index over (y, z)
     if (g.order == 1) {

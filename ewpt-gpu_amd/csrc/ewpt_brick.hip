@@ -54,7 +54,7 @@ int brick_variant_count() { return 11; }
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 11 && v != 9 && order <= 2;
+    return v >= 1 && v <= 11 && v != 9 && order <= 4;
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)

@@ -193,7 +193,8 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 
 /* Kernel variant of the stage launches.  All variants compute the same
  * result; they exist for the autotuner and for measurements.
- *   0 = automatic: the first of 211, 210, 2 that tiles the lattice, else 9
+ *   0 = automatic: the first of 311, 310, 210, 211, 2 that tiles the
+ *       lattice, else 9 (mixed: 311, 211, 310, 210)
  *   1..8, 10, 11 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
  *          5 = 16x8x4 pf1 linear order, 6 = 16x4x4 pf1, 7 = 8x8x4 pf2,
@@ -202,7 +203,8 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *          block order
  *   9 = gather kernel (one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
- * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows), e.g. 111.
+ * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
+ * bricks; 4 = cubes of 4x4x4 bricks), e.g. 111.
  * Setting a brick variant that does not tile the lattice returns
  * EWPT_ERR_UNSUPPORTED. */
 int ewpt_set_variant(ewpt_ctx *ctx, int variant);

@@ -110,7 +110,7 @@ def test_init_modes_matches_cpu_rebuild():
 
 
 @pytest.mark.parametrize("beta", [0.0, 0.2])
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 111, 211])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 111, 211, 310, 311, 410])
 def test_stage_variants_match_oracle(variant, beta):
     shape = (32, 32, 32)
     print(f"variant {variant} beta {beta} shape={shape}", flush=True)
@@ -124,6 +124,21 @@ def test_stage_variants_match_oracle(variant, beta):
     r = cref.rk4(s, p, DX, 3)
     err = rel_l2_per_channel(y, r)
     assert err.max() < 1e-12, (variant, err.argmax(), err.max())
+
+
+@pytest.mark.parametrize("variant", [310, 311, 410, 411])
+def test_cube_orders_match_oracle(variant):
+    # 64x32x32 holds whole cubes of bricks for both brick shapes
+    shape = (64, 32, 32)
+    p = fields.params(gamma=0.2, jex=0.3)
+    s = fields.smooth_state(shape, seed=17, amp=0.5)
+    with ewpt.Lattice(shape, DX) as lat:
+        lat.variant = variant
+        lat.set_state(s)
+        lat.step(p, 2)
+        y = lat.get_state()
+    r = cref.rk4(s, p, DX, 2)
+    assert rel_l2_per_channel(y, r).max() < 1e-12
 
 
 def test_brick_nonuniform_lattice():
@@ -159,7 +174,7 @@ def test_mixed_state_roundtrip():
     assert np.abs(y - s).max() <= 1e-13 * np.abs(s).max()
 
 
-@pytest.mark.parametrize("variant", [0, 10, 11, 210])
+@pytest.mark.parametrize("variant", [0, 10, 11, 210, 311, 410])
 def test_mixed_steps_track_fp64(variant):
     """FP32 storage/stencils with the compensated update stay within the
     stated error of the FP64 oracle: 1e-5 relative L2 per field channel
