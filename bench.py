@@ -234,6 +234,63 @@ def corrector_bench(a) -> int:
     return 0
 
 
+def precision_study(a) -> int:
+    """Mixed path against FP64 from the same thermal IC over a long run: per
+    field-channel relative L2 error, energy drift of each, Gauss-norm ratio.
+    The two states cannot both sit in one GPU's memory at 512^3 (290 GB), so
+    the FP64 run is done first and its final state kept on the host."""
+    import numpy as np
+
+    import ewpt
+    from ewpt import _lib
+
+    n, steps, chunk = a.n, a.study_steps, 100
+    par = ewpt.Parameters(N=n, dx=0.25, gy=0.35, T=0.5)
+    p = par.step_params()
+    out = {}
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        lat = ewpt.Lattice(n, 0.25, precision=_lib.EWPT_FP64 if prec == "fp64" else _lib.EWPT_MIXED)
+        try:
+            lat.init_thermal(0.5, par.meff_sq, par.base_seed, "config")
+            e0, g0 = lat.energies(p), lat.gauss(p)
+            t0 = time.perf_counter()
+            done = 0
+            while done < steps:
+                k = min(chunk, steps - done)
+                lat.step(p, k)
+                done += k
+                print(f"bench: precision study {prec} {done}/{steps} steps ({time.perf_counter() - t0:.0f} s)",
+                      file=sys.stderr, flush=True)
+            wall = time.perf_counter() - t0
+            e1, g1 = lat.energies(p), lat.gauss(p)
+            finals[prec] = lat.get_state()
+        finally:
+            lat.close()
+        out[prec] = {"wall_s": wall, "site_updates_per_s": n ** 3 * steps / wall,
+                     "energy_drift": float((sum(e1[:7]) - sum(e0[:7])) / sum(e0[:7])),
+                     "gauss_l2_start": float(np.sqrt(np.sum(np.asarray(g0[:4]) ** 2))),
+                     "gauss_l2_end": float(np.sqrt(np.sum(np.asarray(g1[:4]) ** 2))),
+                     "aux_l2_end": float(np.sqrt(np.sum(np.asarray(g1[4:]) ** 2)))}
+    ref, mix = finals["fp64"], finals["mixed"]
+    rel = []
+    for c in range(36):
+        den = float(np.linalg.norm(ref[c]))
+        rel.append(float(np.linalg.norm(mix[c] - ref[c]) / den) if den > 0 else float(np.linalg.norm(mix[c])))
+    del finals, ref, mix
+    res = {
+        "metric": f"mixed vs FP64 after {steps} RK4 steps at {n}^3 (thermal IC, T=0.5)",
+        "value": max(rel[:16]), "unit": "max relative L2 error over the 16 field channels",
+        "higher_is_better": False, "n_gpus": 1, "steps": steps,
+        "field_rel_l2": rel[:16], "aux_rel_l2": rel[16:20], "momentum_rel_l2": rel[20:],
+        "speedup_mixed_vs_fp64": out["mixed"]["site_updates_per_s"] / out["fp64"]["site_updates_per_s"],
+        "gauss_ratio_mixed_over_fp64": out["mixed"]["gauss_l2_end"] / out["fp64"]["gauss_l2_end"],
+        "runs": out,
+    }
+    print(json.dumps(res))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,12 +315,18 @@ def main(argv=None) -> int:
     ap.add_argument("--corr-iters", type=int, default=3000)
     ap.add_argument("--corr-lr", type=float, default=3e-4)
     ap.add_argument("--corr-tol", type=float, default=1e-4)
+    ap.add_argument("--precision-study", action="store_true",
+                    help="mixed vs FP64 over --study-steps steps from the same thermal IC (errors, drifts)")
+    ap.add_argument("--study-steps", type=int, default=1000)
     ap.add_argument("--stream-copy", action="store_true",
                     help="only time a plain device copy (HBM ceiling / counter calibration)")
     a = ap.parse_args(argv)
 
     if a.stream_copy:
         return stream_copy()
+
+    if a.precision_study:
+        return precision_study(a)
 
     if a.corrector:
         if a.gpus != 1:

@@ -603,7 +603,7 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
         return EWPT_OK;
     }
     if (mixed(ctx)) {
-        if (variant % 100 != 10 && variant % 100 != 11) return EWPT_ERR_INVAL;
+        if (variant % 100 < 10 || variant % 100 > 13) return EWPT_ERR_INVAL;
         if (!brick_exists(variant)) return EWPT_ERR_INVAL;
         if (!brick_supported_f32(variant, ctx->lat))
             return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");

@@ -236,10 +236,16 @@ struct PassRegs {
 // Prefetch depth of stage S from the profile PF: a single digit applies to
 // all stages, four digits d1 d2 d3 d4 give stages 1..4 (deeper where a
 // halo element is one array, shallower where it is three).
+//
+// A fifth digit 1 (PF >= 10000) pairs the passes of stages 1-3: two channels
+// are staged per LDS barrier (four staging buffers), halving the barriers;
+// stage 4 stays single (its LDS also parks the partial sums).
 __host__ __device__ constexpr int pf_of_stage(int PF, int S)
 {
-    return PF < 10 ? PF : (PF / (S == 1 ? 1000 : S == 2 ? 100 : S == 3 ? 10 : 1)) % 10;
+    return PF >= 10000 ? pf_of_stage(PF % 10000, S)
+         : PF < 10 ? PF : (PF / (S == 1 ? 1000 : S == 2 ? 100 : S == 3 ? 10 : 1)) % 10;
 }
+__host__ __device__ constexpr bool paired_of(int PF, int S) { return PF >= 10000 && S < 4; }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
 // PF: prefetch profile (pf_of_stage).  Two waves per SIMD are requested for
@@ -264,7 +270,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     using R = T;
     // dynamic LDS: two staging buffers, then (stage 4) the 20 partial sums n
     R *lds_base = reinterpret_cast<R *>(lds_raw);
-    R *lds[2] = {lds_base, lds_base + B::BUF};
+    R *lds[4] = {lds_base, lds_base + B::BUF, lds_base + 2 * B::BUF, lds_base + 3 * B::BUF};   // [2..3]: paired
     R *nstash = lds_base + 2 * B::BUF;
 
     // ---- brick of this workgroup
@@ -614,19 +620,46 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     [&]<int... I>(std::integer_sequence<int, I...>) {
         (fetch(std::integral_constant<int, I>{}, rr[I]), ...);
     }(std::make_integer_sequence<int, PFS>{});
-    store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
-    lds_barrier();
-    auto body = [&](auto pc) {
-        constexpr int pp = decltype(pc)::value;
-        if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
-        consume(pc, lds[pp & 1]);
-        pin_all(pc);
-        if constexpr (pp + 1 < NPASS) store(std::integral_constant<int, pp + 1>{}, rr[(pp + 1) % PFS], lds[(pp + 1) & 1]);
+    if constexpr (!paired_of(PF, S)) {
+        store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
         lds_barrier();
-    };
-    [&]<int... I>(std::integer_sequence<int, I...>) {
-        (body(std::integral_constant<int, I>{}), ...);
-    }(std::make_integer_sequence<int, NPASS>{});
+        auto body = [&](auto pc) {
+            constexpr int pp = decltype(pc)::value;
+            if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
+            consume(pc, lds[pp & 1]);
+            pin_all(pc);
+            if constexpr (pp + 1 < NPASS)
+                store(std::integral_constant<int, pp + 1>{}, rr[(pp + 1) % PFS], lds[(pp + 1) & 1]);
+            lds_barrier();
+        };
+        [&]<int... I>(std::integer_sequence<int, I...>) {
+            (body(std::integral_constant<int, I>{}), ...);
+        }(std::make_integer_sequence<int, NPASS>{});
+    } else {
+        // passes 2q, 2q+1 are computed between two barriers from buffers
+        // (2q)%4, (2q+1)%4 while passes 2q+2, 2q+3 are staged into the other two
+        static_assert(PFS >= 3, "paired passes need a prefetch depth of at least 3");
+        store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
+        store(std::integral_constant<int, 1>{}, rr[1], lds[1]);
+        lds_barrier();
+        auto body2 = [&](auto qc) {
+            constexpr int p0 = 2 * decltype(qc)::value, p1 = p0 + 1;
+            if constexpr (p0 + PFS < NPASS) fetch(std::integral_constant<int, p0 + PFS>{}, rr[p0 % PFS]);
+            if constexpr (p1 + PFS < NPASS) fetch(std::integral_constant<int, p1 + PFS>{}, rr[p1 % PFS]);
+            // one pin per pair: the scheduler may interleave the two passes
+            consume(std::integral_constant<int, p0>{}, lds[p0 & 3]);
+            consume(std::integral_constant<int, p1>{}, lds[p1 & 3]);
+            pin_all(std::integral_constant<int, p1>{});
+            if constexpr (p0 + 2 < NPASS)
+                store(std::integral_constant<int, p0 + 2>{}, rr[(p0 + 2) % PFS], lds[(p0 + 2) & 3]);
+            if constexpr (p1 + 2 < NPASS)
+                store(std::integral_constant<int, p1 + 2>{}, rr[(p1 + 2) % PFS], lds[(p1 + 2) & 3]);
+            lds_barrier();
+        };
+        [&]<int... I>(std::integer_sequence<int, I...>) {
+            (body2(std::integral_constant<int, I>{}), ...);
+        }(std::make_integer_sequence<int, NPASS / 2>{});
+    }
 
     // ---- outputs: k_Gamma(3), k_Sigma, kt_phi(4), kt_W(9), kt_Y(3)
     R o[NK];
@@ -688,13 +721,18 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
 {
     using B = BrickShape<TX, TY, TZ>;
     const dim3 grid(g.nbx * g.nby * g.nbz), block(B::NT);
-    const size_t lds13 = 2 * B::BUF * sizeof(T);
-    // stage 4 parks n (FP64) or the increment and the high word (mixed)
-    const size_t lds4 = lds13 + (std::is_same_v<T, float> ? 2 : 1) * NK * B::NT * sizeof(T);
-    static bool attr_set = false;   // the stage-4 kernels may need more than 64 KB of LDS
+    const size_t lds13 = (paired_of(PF, 1) ? 4 : 2) * B::BUF * sizeof(T);
+    // stage 4 (never paired) parks n (FP64) or the increment and the high word (mixed)
+    const size_t lds4 = 2 * B::BUF * sizeof(T) + (std::is_same_v<T, float> ? 2 : 1) * NK * B::NT * sizeof(T);
+    static bool attr_set = false;   // the kernels may need more than 64 KB of LDS
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4, T>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
+        const void *k13[3] = {reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 1, T>),
+                              reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 2, T>),
+                              reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 3, T>)};
+        for (int i = 0; i < 3 && e == hipSuccess; i++)
+            e = hipFuncSetAttribute(k13[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds13);
         if (e != hipSuccess) return e;
         attr_set = true;
     }

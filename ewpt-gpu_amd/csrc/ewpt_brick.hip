@@ -12,6 +12,7 @@
 //   8: 16x4x4 brick, 256 threads, prefetch distance 2, XCD-aware block order
 //   10: 16x8x4 brick, 512 threads, prefetch depth 4/4/3/2 by stage, XCD-aware block order
 //   11: 32x4x4 brick, 512 threads, prefetch depth 4/4/3/2 by stage, XCD-aware block order
+//   12, 13: as 10, 11 with two channels per LDS barrier in stages 1-3
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -49,12 +50,20 @@ bool brick_fits_v11(const ewpt_lattice &L);
 hipError_t launch_brick_v11(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
-int brick_variant_count() { return 11; }
+bool brick_fits_v12(const ewpt_lattice &L);
+hipError_t launch_brick_v12(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v13(const ewpt_lattice &L);
+hipError_t launch_brick_v13(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+int brick_variant_count() { return 13; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 11 && v != 9 && order <= 4;
+    return v >= 1 && v <= 13 && v != 9 && order <= 4;
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)
@@ -70,6 +79,8 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 8: return brick_fits_v8(L);
     case 10: return brick_fits_v10(L);
     case 11: return brick_fits_v11(L);
+    case 12: return brick_fits_v12(L);
+    case 13: return brick_fits_v13(L);
     default: return false;
     }
 }
@@ -90,6 +101,8 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 8: return launch_brick_v8(stage, b, L, P, dt, order, st);
     case 10: return launch_brick_v10(stage, b, L, P, dt, order, st);
     case 11: return launch_brick_v11(stage, b, L, P, dt, order, st);
+    case 12: return launch_brick_v12(stage, b, L, P, dt, order, st);
+    case 13: return launch_brick_v13(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -2,6 +2,7 @@
 // arithmetic with the compensated stage-4 update (ewpt_brick.h, T = float).
 //   10 (+100*order): 16x8x4 brick, prefetch depth 4/4/3/2
 //   11 (+100*order): 32x4x4 brick, prefetch depth 4/4/3/2
+//   12, 13 (+100*order): as 10, 11 with two channels per LDS barrier in stages 1-3
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -11,6 +12,8 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     switch (variant % 100) {
     case 10: return brick_fits<16, 8, 4>(L);
     case 11: return brick_fits<32, 4, 4>(L);
+    case 12: return brick_fits<16, 8, 4>(L);
+    case 13: return brick_fits<32, 4, 4>(L);
     default: return false;
     }
 }
@@ -22,6 +25,8 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     switch (variant % 100) {
     case 10: return launch_tile<16, 8, 4, 4432, float>(stage, b, L, P, dt, 1, order, st);
     case 11: return launch_tile<32, 4, 4, 4432, float>(stage, b, L, P, dt, 1, order, st);
+    case 12: return launch_tile<16, 8, 4, 14432, float>(stage, b, L, P, dt, 1, order, st);
+    case 13: return launch_tile<32, 4, 4, 14432, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }

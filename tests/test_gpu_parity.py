@@ -110,7 +110,7 @@ def test_init_modes_matches_cpu_rebuild():
 
 
 @pytest.mark.parametrize("beta", [0.0, 0.2])
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 111, 211, 310, 311, 410])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 111, 211, 310, 311, 410, 312])
 def test_stage_variants_match_oracle(variant, beta):
     shape = (32, 32, 32)
     print(f"variant {variant} beta {beta} shape={shape}", flush=True)
@@ -174,7 +174,7 @@ def test_mixed_state_roundtrip():
     assert np.abs(y - s).max() <= 1e-13 * np.abs(s).max()
 
 
-@pytest.mark.parametrize("variant", [0, 10, 11, 210, 311, 410])
+@pytest.mark.parametrize("variant", [0, 10, 11, 12, 13, 210, 311, 410])
 def test_mixed_steps_track_fp64(variant):
     """FP32 storage/stencils with the compensated update stay within the
     stated error of the FP64 oracle: 1e-5 relative L2 per field channel
