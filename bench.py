@@ -57,8 +57,14 @@ def cpu_baseline(n: int = 128, steps: int = 16):
 
 
 def stream_copy(nbytes: int = 4 << 30, reps: int = 10) -> int:
-    """Time a plain device-to-device copy of nbytes (read + write) with torch."""
+    """Time plain device-to-device copies of nbytes (read + write): torch's
+    copy_ and libewpt's one-8-byte-access-per-lane copy kernel (the stage
+    kernels' access width), for the HBM ceiling and counter calibration."""
+    import ctypes
+
     import torch
+
+    import ewpt
     n = nbytes // 8
     src = torch.rand(n, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
@@ -71,8 +77,15 @@ def stream_copy(nbytes: int = 4 << 30, reps: int = 10) -> int:
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
+    ms2 = ctypes.c_double(0.0)
+    rc = ewpt.load().ewpt_copy_probe(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()), n, reps,
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), ctypes.byref(ms2))
+    if rc != 0:
+        raise RuntimeError(f"ewpt_copy_probe failed ({rc})")
     print(json.dumps({"stream_copy_bytes_read": nbytes, "stream_copy_bytes_written": nbytes,
-                      "ms": ms, "GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+                      "torch_copy_ms": ms, "torch_copy_GB_per_s": 2 * nbytes / (ms * 1e-3) / 1e9,
+                      "fp64_lane_copy_ms": ms2.value,
+                      "fp64_lane_copy_GB_per_s": 2 * nbytes / (ms2.value * 1e-3) / 1e9}))
     return 0
 
 

@@ -289,6 +289,26 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host)
     return EWPT_OK;
 }
 
+int ewpt_copy_probe(const double *dev_src, double *dev_dst, long n, int reps, void *stream, double *ms)
+{
+    if (!dev_src || !dev_dst || n < 1 || reps < 1 || !ms) return EWPT_ERR_INVAL;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return EWPT_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return EWPT_ERR_HIP; }
+    hipError_t e = launch_copy(dev_src, dev_dst, n, st);   // untimed first launch
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(dev_src, dev_dst, n, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = f / reps;
+    return e == hipSuccess ? EWPT_OK : EWPT_ERR_HIP;
+}
+
 int ewpt_copy_state(ewpt_ctx *dst, ewpt_ctx *src)
 {
     if (!dst || !src || dst == src) return EWPT_ERR_INVAL;

@@ -57,6 +57,7 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
 // ewpt_mixed.hip
 hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream_t st);
 hipError_t launch_join(const float *hi, const float *lo, double *y, long n, hipStream_t st);
+hipError_t launch_copy(const double *a, double *b, long n, hipStream_t st);
 
 // stage kernel variants selectable through ewpt_set_variant
 constexpr int VARIANT_AUTO = 0;

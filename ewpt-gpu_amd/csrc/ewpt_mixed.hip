@@ -28,6 +28,13 @@ __global__ void __launch_bounds__(256) join_kernel(const float *hi, const float 
         y[i] = (double)hi[i] + (double)lo[i];
 }
 
+// plain FP64 copy, one 8-byte element per lane and iteration (the access
+// width of the stage kernels): the reference point for memory counters
+__global__ void __launch_bounds__(256) copy_kernel(const double *__restrict__ a, double *__restrict__ b, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) b[i] = a[i];
+}
+
 unsigned conv_grid(long n)
 {
     const long b = (n + 255) / 256;
@@ -45,6 +52,12 @@ hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream
 hipError_t launch_join(const float *hi, const float *lo, double *y, long n, hipStream_t st)
 {
     join_kernel<<<conv_grid(n), 256, 0, st>>>(hi, lo, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const double *a, double *b, long n, hipStream_t st)
+{
+    copy_kernel<<<conv_grid(n), 256, 0, st>>>(a, b, n);
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ EXPORTS = (
     "ewpt_copy_state", "ewpt_net_create", "ewpt_net_destroy", "ewpt_net_last_error", "ewpt_net_param_count",
     "ewpt_net_set_params", "ewpt_net_get_params", "ewpt_net_get_grads", "ewpt_net_set_norm", "ewpt_net_apply",
     "ewpt_net_train_step", "ewpt_step_corrected", "ewpt_net_conv_test", "ewpt_net_wgrad_test",
-    "ewpt_net_set_input",
+    "ewpt_net_set_input", "ewpt_copy_probe",
 )
 
 
@@ -104,6 +104,7 @@ def load() -> ctypes.CDLL:
         "ewpt_batch_step_timed": [vp, pp, ip, dp],
         "ewpt_batch_tune": [vp, pp, ip, ctypes.POINTER(ctypes.c_int), dp],
         "ewpt_copy_state": [vp, vp],
+        "ewpt_copy_probe": [vp, vp, ctypes.c_long, ip, vp, dp],
         "ewpt_net_create": [ctypes.POINTER(vp), ip, ctypes.c_ulonglong],
         "ewpt_net_destroy": [vp],
         "ewpt_net_set_params": [vp, fp],

@@ -67,6 +67,12 @@ int ewpt_destroy(ewpt_ctx *ctx);
 int ewpt_set_state(ewpt_ctx *ctx, const double *host_state);
 int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
 
+/* Measurement aid, no counterpart in the reference: a plain FP64 device copy
+ * of n doubles with one 8-byte access per lane (the stage kernels' access
+ * width), launched reps + 1 times on stream; *ms = mean time of the last
+ * reps.  The reference point for memory-traffic counters. */
+int ewpt_copy_probe(const double *dev_src, double *dev_dst, long n, int reps, void *stream, double *ms);
+
 /* Device-to-device copy of the state of src into dst (same device,
  * precision and lattice). */
 int ewpt_copy_state(ewpt_ctx *dst, ewpt_ctx *src);
