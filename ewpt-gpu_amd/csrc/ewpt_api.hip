@@ -31,6 +31,13 @@ struct ewpt_ctx {
     double *y = nullptr, *ynext = nullptr, *k[3] = {nullptr, nullptr, nullptr};
     float *fhi = nullptr, *flo = nullptr, *fhi_n = nullptr, *flo_n = nullptr;
     float *fk[3] = {nullptr, nullptr, nullptr};
+    // stage-input buffer of the stored-stage-input variants
+    // (brick_stored_inputs), allocated when such a variant is first used: u2
+    // and u4 live here, u3 in the next-state buffer, which stage 4 only
+    // writes after stage 3 has read it
+    double *u = nullptr;
+    float *fu = nullptr;
+    bool stored_ok = true;   // false: no room for it, AUTO skips those variants
     bool view_valid = true;
     // host-buffer RHS path (allocated on first use)
     double *rhs_in = nullptr, *rhs_out = nullptr;
@@ -80,10 +87,11 @@ int dev_alloc(ewpt_ctx *c, T **p, size_t n, const char *what)
 
 void release(ewpt_ctx *c)
 {
-    double **bufs[] = {&c->y, &c->ynext, &c->k[0], &c->k[1], &c->k[2], &c->rhs_in, &c->rhs_out, &c->part, &c->res};
+    double **bufs[] = {&c->y, &c->ynext, &c->k[0], &c->k[1], &c->k[2], &c->rhs_in, &c->rhs_out, &c->part, &c->res,
+                       &c->u};
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
-    float **fbufs[] = {&c->fhi, &c->flo, &c->fhi_n, &c->flo_n, &c->fk[0], &c->fk[1], &c->fk[2]};
+    float **fbufs[] = {&c->fhi, &c->flo, &c->fhi_n, &c->flo_n, &c->fk[0], &c->fk[1], &c->fk[2], &c->fu};
     for (float **b : fbufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (c->stream) { (void)hipStreamDestroy(c->stream); c->stream = nullptr; }
@@ -104,7 +112,16 @@ bool lattice_ok(const ewpt_lattice *L)
            (long)L->nx * L->ny <= (1L << 30) && (long)L->nx * L->ny * (long)L->nz <= (1L << 34);
 }
 
-StageBufs<double> bufs_for(ewpt_ctx *c, int stage)
+// stage inputs of the stored-stage-input variants: u2 -> U, u3 -> the
+// next-state buffer, u4 -> U (stage 4 then writes the next state)
+template <typename T>
+void set_stage_inputs(StageBufs<T> &b, int stage, T *U, T *next)
+{
+    b.u = stage == 3 ? next : stage == 1 ? nullptr : U;
+    b.unext = stage == 2 ? next : stage == 4 ? nullptr : U;
+}
+
+StageBufs<double> bufs_for(ewpt_ctx *c, int stage, bool ustaged = false)
 {
     StageBufs<double> b{};
     b.y = c->y;
@@ -114,7 +131,16 @@ StageBufs<double> bufs_for(ewpt_ctx *c, int stage)
     b.n3 = c->n3;
     if (stage < 4) b.kout = c->k[stage - 1];
     else b.ynew = c->ynext;
+    if (ustaged) set_stage_inputs(b, stage, c->u, c->ynext);
     return b;
+}
+
+int ensure_stage_input_buffer(ewpt_ctx *c, int variant)
+{
+    if (!brick_stored_inputs(variant)) return EWPT_OK;
+    const size_t n = NCH * (size_t)c->n3;
+    if (c->precision == EWPT_MIXED) return c->fu ? EWPT_OK : dev_alloc(c, &c->fu, n, "alloc stage input (fp32)");
+    return c->u ? EWPT_OK : dev_alloc(c, &c->u, n, "alloc stage input");
 }
 
 int set_device(ewpt_ctx *c) { HIPCHK(c, hipSetDevice(c->device), "hipSetDevice"); return EWPT_OK; }
@@ -140,15 +166,17 @@ int push_view(ewpt_ctx *c)
 }
 
 // variant actually launched: AUTO takes the first brick variant that tiles
-// the lattice from the measured ranking (512^3 on MI355X, FP64: 311 186.8,
-// 310 187.9, 210 190.0, 211 193.8 ms/step; mixed: 311 97.1, 211 103.0,
-// 310 133.9 ms/step), else the gather kernel
+// the lattice from the measured ranking (512^3 on MI355X, FP64: 311 185.3,
+// 310 187.0, 210 190.2, 211 193.8 ms/step; mixed: 315 90.7, 215 92.1,
+// 311 96.5, 211 103.2, 310 133.9 ms/step), else the gather kernel.  The
+// stored-stage-input variants gain 1% in FP64 for 36 more channels of
+// memory, so FP64 AUTO does not use them; mixed AUTO does when the buffer fits.
 int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
     if (mixed(c)) {
-        for (int v : {311, 211, 310, 210})
-            if (brick_supported_f32(v, c->lat)) return v;
+        for (int v : {315, 215, 311, 211, 310, 210})
+            if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported_f32(v, c->lat)) return v;
         return -1;
     }
     for (int v : {311, 310, 210, 211, 2})
@@ -160,6 +188,11 @@ int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
 {
     const Phys P = phys_of(*p);
     const int v = resolved_variant(c);
+    const bool us = brick_stored_inputs(v);
+    if (us && stage == 1) {
+        const int rc = ensure_stage_input_buffer(c, v);
+        if (rc) return rc;
+    }
     if (mixed(c)) {
         StageBufs<float> b{};
         b.y = c->fhi;
@@ -170,6 +203,7 @@ int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
         b.n3 = c->n3;
         if (stage < 4) b.kout = c->fk[stage - 1];
         else { b.ynew = c->fhi_n; b.ylo_new = c->flo_n; }
+        if (us) set_stage_inputs(b, stage, c->fu, c->fhi_n);
         HIPCHK(c, launch_stage_brick_f32(v, stage, b, c->lat, P, p->dt, c->stream), "stage launch");
         if (stage == 4) {
             std::swap(c->fhi, c->fhi_n);
@@ -181,7 +215,7 @@ int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
     if (v == VARIANT_GATHER)
         HIPCHK(c, launch_stage_gather(stage, bufs_for(c, stage), c->lat, P, p->dt, c->stream), "stage launch");
     else
-        HIPCHK(c, launch_stage_brick(v, stage, bufs_for(c, stage), c->lat, P, p->dt, c->stream), "stage launch");
+        HIPCHK(c, launch_stage_brick(v, stage, bufs_for(c, stage, us), c->lat, P, p->dt, c->stream), "stage launch");
     if (stage == 4) std::swap(c->y, c->ynext);
     return EWPT_OK;
 }
@@ -244,6 +278,16 @@ int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devi
     c->nred = reduce_blocks(c->lat);
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->part, (size_t)c->nred * NRED_SLOTS, "alloc partials");
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->res, NRED_SLOTS, "alloc results");
+    if (rc == EWPT_OK && brick_stored_inputs(resolved_variant(c))) {
+        // AUTO's first choice wants the stage-input buffer: take it now, or
+        // fall back to the variants that do without
+        rc = ensure_stage_input_buffer(c, resolved_variant(c));
+        if (rc == EWPT_ERR_OOM) {
+            c->stored_ok = false;
+            c->err.clear();
+            rc = EWPT_OK;
+        }
+    }
     if (rc != EWPT_OK) {
         release(c);
         delete c;
@@ -623,10 +667,11 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
         return EWPT_OK;
     }
     if (mixed(ctx)) {
-        if (variant % 100 < 10 || variant % 100 > 13) return EWPT_ERR_INVAL;
+        if (variant % 100 < 10 || variant % 100 > 19) return EWPT_ERR_INVAL;
         if (!brick_exists(variant)) return EWPT_ERR_INVAL;
         if (!brick_supported_f32(variant, ctx->lat))
             return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
+        if (int rc = ensure_stage_input_buffer(ctx, variant)) return rc;
         ctx->variant = variant;
         return EWPT_OK;
     }
@@ -637,6 +682,7 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
     if (!brick_exists(variant)) return EWPT_ERR_INVAL;
     if (!brick_supported(variant, ctx->lat))
         return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
+    if (int rc = ensure_stage_input_buffer(ctx, variant)) return rc;
     ctx->variant = variant;
     return EWPT_OK;
 }

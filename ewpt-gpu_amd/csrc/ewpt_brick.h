@@ -242,10 +242,17 @@ struct PassRegs {
 // stage 4 stays single (its LDS also parks the partial sums).
 __host__ __device__ constexpr int pf_of_stage(int PF, int S)
 {
-    return PF >= 10000 ? pf_of_stage(PF % 10000, S)
+    return PF >= 100000 ? pf_of_stage(PF % 100000, S)
+         : PF >= 10000 ? pf_of_stage(PF % 10000, S)
          : PF < 10 ? PF : (PF / (S == 1 ? 1000 : S == 2 ? 100 : S == 3 ? 10 : 1)) % 10;
 }
-__host__ __device__ constexpr bool paired_of(int PF, int S) { return PF >= 10000 && S < 4; }
+__host__ __device__ constexpr bool paired_of(int PF, int S) { return PF % 100000 >= 10000 && S < 4; }
+// PF / 100000 = m: stored stage inputs.  Bit s-2 of m set: stage s (2..4)
+// reads its input u_s as one array (every staged value, centre or halo, is a
+// single load instead of two or three), and stage s-1 writes it (36 channels:
+// y + c k_{s-1}, the fields' k being the stage momenta).
+__host__ __device__ constexpr bool stored_in(int PF, int S) { return S >= 2 && ((PF / 100000) >> (S - 2)) & 1; }
+__host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
 // PF: prefetch profile (pf_of_stage).  Two waves per SIMD are requested for
@@ -330,11 +337,13 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     }
 
     // ---- centre values of the stage input (ewpt_stage.h)
+    constexpr bool US = stored_in(PF, S);
+    auto U = [&](int ch) { return b.u + (long)ch * b.n3; };
     R c[NCH];
 #pragma unroll
     for (int ch = 0; ch < NCH; ch++) {
-        const R yv = ldo(Y(ch), qb);
-        if constexpr (S == 1) {
+        const R yv = ldo(US ? U(ch) : Y(ch), qb);
+        if constexpr (S == 1 || US) {
             c[ch] = yv;
         } else if (ch < 16) {
             const R piv = ldo(Y(ch + 20), qb);
@@ -414,9 +423,9 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         constexpr int ch = pass_channel(pp);
         constexpr int dir = pass_dir(pp);
         constexpr int nf = face_count<TX, TY, TZ>(dir);
-        const T *y0p = Y(ch);
+        const T *y0p = US ? U(ch) : Y(ch);
         const T *y1p = nullptr, *k2p = nullptr;
-        if constexpr (S > 1) {
+        if constexpr (S > 1 && !US) {
             if constexpr (ch < 16) {
                 y1p = Y(ch + 20);
                 if constexpr (S == 3) k2p = K(b.k1, ch + 4);
@@ -431,14 +440,14 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             if (s * B::NT < nf) {
                 const unsigned o = (unsigned)goff[dir][s];
                 rg.v[s][0] = ldo(y0p, o);
-                if constexpr (S > 1) rg.v[s][1] = ldo(y1p, o);
-                if constexpr (S > 2 && ch < 16) rg.v[s][2] = ldo(k2p, o);
+                if constexpr (S > 1 && !US) rg.v[s][1] = ldo(y1p, o);
+                if constexpr (S > 2 && !US && ch < 16) rg.v[s][2] = ldo(k2p, o);
             }
         }
     };
     auto stage_value = [&](auto pc, const T (&v)[3]) -> R {
         constexpr int ch = pass_channel(decltype(pc)::value);
-        if constexpr (S == 1) return v[0];
+        if constexpr (S == 1 || US) return v[0];
         else if constexpr (ch < 16) {
             if constexpr (S == 2) return v[0] + v[1] * half_dt;
             else if constexpr (S == 3) return v[0] + (v[1] + v[2] * half_dt) * half_dt;
@@ -678,6 +687,17 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     if constexpr (S < 4) {
 #pragma unroll
         for (int m = 0; m < NK; m++) sto(b.kout + (long)m * b.n3, qb, o[m]);
+        if constexpr (stores_next(PF, S)) {
+            // next stage input: fields y + c * (this stage's momentum input),
+            // the other channels y + c * k_s (the expressions of stage_in)
+            const R cc = S == 3 ? dt : half_dt;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ch++) {
+                const R yv = S == 1 ? c[ch] : R(ldo(Y(ch), qb));
+                const R kv = ch < 16 ? c[ch + 20] : o[ch - 16];
+                sto(b.unext + (long)ch * b.n3, qb, yv + kv * cc);
+            }
+        }
     } else if constexpr (MIXED) {
 #pragma unroll
         for (int m = 0; m < NK; m++) {

@@ -13,6 +13,9 @@
 //   10: 16x8x4 brick, 512 threads, prefetch depth 4/4/3/2 by stage, XCD-aware block order
 //   11: 32x4x4 brick, 512 threads, prefetch depth 4/4/3/2 by stage, XCD-aware block order
 //   12, 13: as 10, 11 with two channels per LDS barrier in stages 1-3
+//   14, 15: as 10, 11 with stored stage inputs for stages 2-4 (brick_stored_inputs)
+//   16, 17: as 10, 11 with a stored stage input for stage 4
+//   18, 19: as 10, 11 with stored stage inputs for stages 3-4
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -58,12 +61,42 @@ bool brick_fits_v13(const ewpt_lattice &L);
 hipError_t launch_brick_v13(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
-int brick_variant_count() { return 13; }
+bool brick_fits_v14(const ewpt_lattice &L);
+hipError_t launch_brick_v14(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v15(const ewpt_lattice &L);
+hipError_t launch_brick_v15(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v16(const ewpt_lattice &L);
+hipError_t launch_brick_v16(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v17(const ewpt_lattice &L);
+hipError_t launch_brick_v17(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v18(const ewpt_lattice &L);
+hipError_t launch_brick_v18(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v19(const ewpt_lattice &L);
+hipError_t launch_brick_v19(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+int brick_variant_count() { return 19; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 13 && v != 9 && order <= 4;
+    return v >= 1 && v <= 19 && v != 9 && order <= 4;
+}
+
+bool brick_stored_inputs(int variant)
+{
+    const int v = variant % 100;
+    return variant >= 0 && v >= 14 && v <= 19;
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)
@@ -81,6 +114,12 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 11: return brick_fits_v11(L);
     case 12: return brick_fits_v12(L);
     case 13: return brick_fits_v13(L);
+    case 14: return brick_fits_v14(L);
+    case 15: return brick_fits_v15(L);
+    case 16: return brick_fits_v16(L);
+    case 17: return brick_fits_v17(L);
+    case 18: return brick_fits_v18(L);
+    case 19: return brick_fits_v19(L);
     default: return false;
     }
 }
@@ -103,6 +142,12 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 11: return launch_brick_v11(stage, b, L, P, dt, order, st);
     case 12: return launch_brick_v12(stage, b, L, P, dt, order, st);
     case 13: return launch_brick_v13(stage, b, L, P, dt, order, st);
+    case 14: return launch_brick_v14(stage, b, L, P, dt, order, st);
+    case 15: return launch_brick_v15(stage, b, L, P, dt, order, st);
+    case 16: return launch_brick_v16(stage, b, L, P, dt, order, st);
+    case 17: return launch_brick_v17(stage, b, L, P, dt, order, st);
+    case 18: return launch_brick_v18(stage, b, L, P, dt, order, st);
+    case 19: return launch_brick_v19(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -3,6 +3,9 @@
 //   10 (+100*order): 16x8x4 brick, prefetch depth 4/4/3/2
 //   11 (+100*order): 32x4x4 brick, prefetch depth 4/4/3/2
 //   12, 13 (+100*order): as 10, 11 with two channels per LDS barrier in stages 1-3
+//   14, 15 (+100*order): as 10, 11 with stored stage inputs for stages 2-4
+//   16, 17 (+100*order): as 10, 11 with a stored stage input for stage 4
+//   18, 19 (+100*order): as 10, 11 with stored stage inputs for stages 3-4
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -14,6 +17,12 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 11: return brick_fits<32, 4, 4>(L);
     case 12: return brick_fits<16, 8, 4>(L);
     case 13: return brick_fits<32, 4, 4>(L);
+    case 14: return brick_fits<16, 8, 4>(L);
+    case 15: return brick_fits<32, 4, 4>(L);
+    case 16: return brick_fits<16, 8, 4>(L);
+    case 17: return brick_fits<32, 4, 4>(L);
+    case 18: return brick_fits<16, 8, 4>(L);
+    case 19: return brick_fits<32, 4, 4>(L);
     default: return false;
     }
 }
@@ -27,6 +36,12 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 11: return launch_tile<32, 4, 4, 4432, float>(stage, b, L, P, dt, 1, order, st);
     case 12: return launch_tile<16, 8, 4, 14432, float>(stage, b, L, P, dt, 1, order, st);
     case 13: return launch_tile<32, 4, 4, 14432, float>(stage, b, L, P, dt, 1, order, st);
+    case 14: return launch_tile<16, 8, 4, 704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 15: return launch_tile<32, 4, 4, 704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 16: return launch_tile<16, 8, 4, 404432, float>(stage, b, L, P, dt, 1, order, st);
+    case 17: return launch_tile<32, 4, 4, 404432, float>(stage, b, L, P, dt, 1, order, st);
+    case 18: return launch_tile<16, 8, 4, 604432, float>(stage, b, L, P, dt, 1, order, st);
+    case 19: return launch_tile<32, 4, 4, 604432, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }

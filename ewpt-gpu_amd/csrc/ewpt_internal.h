@@ -45,6 +45,9 @@ long distinct_k2_count(int n0, int n1, int n2);
 // ewpt_brick.hip -- LDS-tiled fused stage kernels, variants 1..5
 int brick_variant_count();
 bool brick_exists(int variant);
+// variants in which some of stages 2-4 read their input as one array written
+// by the stage before (StageBufs::u / unext)
+bool brick_stored_inputs(int variant);
 bool brick_supported(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
                               const Phys &P, double dt, hipStream_t st);

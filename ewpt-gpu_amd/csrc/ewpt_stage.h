@@ -8,7 +8,9 @@
 //   k1_f = pi,  k2_f = pi + kt1*(dt/2),  k3_f = pi + kt2*(dt/2),  k4_f = pi + kt3*dt.
 // Stage inputs are y + k1*(dt/2), y + k2*(dt/2), y + k3*dt (6226, 6298, 6370)
 // and the update is n = k1*(dt/6) + y; n += k2*(dt/3); n += k3*(dt/3);
-// y' = k4*(dt/6) + n (6183-6437).
+// y' = k4*(dt/6) + n (6183-6437).  The stage kernels either form each stage
+// input from y and the stored k's when they load it, or (stored stage inputs)
+// receive it as one array u written by the previous stage.
 #pragma once
 
 #include "ewpt_point.h"
@@ -26,6 +28,10 @@ struct StageBufs {
     const T *ylo;   // mixed precision: compensation word of y (36 * n3) or null
     T *ylo_new;
     long n3;
+    // kernels that pass the stage input on (stored stage inputs): u is this
+    // stage's input (36 * n3; stage 1 reads y), unext receives the next one
+    const T *u;
+    T *unext;
 };
 
 // Value of channel c of the stage-S input at flat site q.

@@ -198,15 +198,22 @@ int ewpt_spectrum(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int field
 int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 
 /* Kernel variant of the stage launches.  All variants compute the same
- * result; they exist for the autotuner and for measurements.
+ * result (to the rounding of the arithmetic); they exist for the autotuner
+ * and for measurements.
  *   0 = automatic: the first of 311, 310, 210, 211, 2 that tiles the
- *       lattice, else 9 (mixed: 311, 211, 310, 210)
- *   1..8, 10, 11 = LDS-brick stage kernel (brick, prefetch, block order):
+ *       lattice, else 9 (mixed: 315, 215, 311, 211, 310, 210; 315 and 215
+ *       only when the stage-input buffer fits in device memory)
+ *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
  *          5 = 16x8x4 pf1 linear order, 6 = 16x4x4 pf1, 7 = 8x8x4 pf2,
  *          8 = 16x4x4 pf2, 10 = 16x8x4 and 11 = 32x4x4 with prefetch
- *          depth 4/4/3/2 for stages 1..4; all but 5 use the XCD-aware
- *          block order
+ *          depth 4/4/3/2 for stages 1..4; 12, 13 = 10, 11 with two channels
+ *          per LDS barrier; 14, 15 = 10, 11 with stored stage inputs
+ *          (stages 1-3 also write the next stage's input, 36 channels,
+ *          which stages 2-4 read as one array); 16, 17 = the same for
+ *          stage 4 only; 18, 19 = for stages 3 and 4.  14..19 hold one more
+ *          state-sized buffer (allocated when first selected).  All but 5
+ *          use the XCD-aware block order.  The mixed path takes 10..19.
  *   9 = gather kernel (one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
