@@ -29,17 +29,48 @@ for _p in (os.path.join(ROOT, "ewpt-gpu_amd"), ROOT):
 
 METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline"
 HBM_PEAK = 8.0e12
-# Algorithmic HBM bytes per site per stage launch (stage-separated RK4 with 20
-# stored derivative channels per stage), as channel accesses times the word
-# size.  FP64: S1 reads y, writes k1; S2 reads y, k1, writes k2; S3 reads y,
-# the 16 momentum k1's, k2, writes k3; S4 reads y, k1..k3, writes y'.  Mixed:
-# the same in FP32 plus the FP32 compensation word of y, read and written at
-# stage 4 only.
-STAGE_BYTES = {
-    "fp64": {1: (36 + 20) * 8, 2: (36 + 20 + 20) * 8, 3: (36 + 16 + 20 + 20) * 8, 4: (36 + 60 + 36) * 8},
-    "mixed": {1: (36 + 20) * 4, 2: (36 + 20 + 20) * 4, 3: (36 + 16 + 20 + 20) * 4,
-              4: (36 + 36 + 60 + 36 + 36) * 4},
-}
+# Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
+# reads its input as one array written by stage s-1.
+STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6}
+
+
+def stage_bytes_per_site(precision: str, variant: int) -> dict:
+    """Algorithmic HBM bytes per site of each stage launch: channel accesses
+    times the word size (8 FP64, 4 mixed).  Formed stage inputs: S1 reads y,
+    writes k1; S2 reads y, k1, writes k2; S3 reads y, the 16 momentum k1's,
+    k2, writes k3; S4 reads y, k1..k3, writes y'.  A stored input u_s
+    replaces the formed reads of stage s by one 36-channel array, and stage
+    s-1 then also writes u_s (reading y for it).  Mixed adds the FP32
+    compensation word of y, read and written at stage 4.  Variant 0 or any
+    non-stored variant gives the minimal (formed) figures."""
+    m = STORED_MASK.get(variant % 100, 0) if variant > 0 else 0
+    st = {s: s >= 2 and bool((m >> (s - 2)) & 1) for s in (1, 2, 3, 4, 5)}
+    formed = {1: 36, 2: 36 + 20, 3: 36 + 16 + 20}
+    ch = {}
+    for s in (1, 2, 3):
+        reads = 36 if st[s] else formed[s]
+        if st[s + 1] and st[s]:
+            reads += 36                      # y for the next stage input
+        ch[s] = reads + 20 + (36 if st[s + 1] else 0)
+    ch[4] = 36 + 60 + (36 if st[4] else 0) + 36
+    word = 8 if precision == "fp64" else 4
+    if precision != "fp64":
+        ch[4] += 36 + 36
+    return {s: c * word for s, c in ch.items()}
+
+
+STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
+
+# Bytes per site that one stage launch moves across the L2/fabric boundary
+# at 512^3, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (profiles/stage_traffic_512.md): FETCH_SIZE x 2 (what that counter reads
+# for a plain device copy of known size) + WRITE_SIZE x 1.
+MEASURED_TRAFFIC = {}
+
+
+def measured_traffic(precision: str, variant: int, stage: int, n: int):
+    b = MEASURED_TRAFFIC.get((precision, variant, stage)) if n == 512 else None
+    return None if b is None else b * n ** 3
 
 
 def cpu_baseline(n: int = 128, steps: int = 16):
@@ -382,20 +413,24 @@ def main(argv=None) -> int:
             if a.warmup:
                 lat.step(P, a.warmup)
             stage_ms, total_ms = lat.step_timed(P, a.steps)   # synchronises before returning
+            active = lat.active_variant
             e = lat.energies(P)
         finally:
             lat.close()
         value = sites * a.steps / (total_ms * 1e-3)
-        stage_bytes = {s: STAGE_BYTES[precision][s] * sites for s in (1, 2, 3, 4)}
+        per_site = stage_bytes_per_site(precision, active)
+        stage_bytes = {s: per_site[s] * sites for s in (1, 2, 3, 4)}
         dom = max(range(4), key=lambda s: stage_ms[s]) + 1
         achieved = stage_bytes[dom] / (stage_ms[dom - 1] * 1e-3)
         return {
             "value": value,
             "ms_per_step": total_ms / a.steps,
+            "variant": active,
             "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK, "traffic": None,
-                         "kernel": f"stage {dom} ({precision})", "bytes_per_site": stage_bytes[dom] // sites},
-            "step_roofline_frac": value * sum(stage_bytes.values()) / sites / HBM_PEAK,
+                         "frac": achieved / HBM_PEAK, "traffic": measured_traffic(precision, active, dom, n),
+                         "kernel": f"stage {dom} ({precision}, variant {active})", "bytes_per_site": per_site[dom]},
+            # the step against the minimal (formed-input) traffic of an RK4 step
+            "step_roofline_frac": value * sum(STAGE_BYTES[precision].values()) / HBM_PEAK,
             "stage_ms": stage_ms,
             "energy_check": float(sum(e[:7])),
         }
@@ -419,7 +454,7 @@ def main(argv=None) -> int:
                  if a.ic == "thermal" else "synthetic: seeded smooth Fourier-mode fields generated on the device"),
         "config": {"workload": f"RK4 step of the 36-channel Higgs-gauge lattice, {n}^3 periodic, "
                                f"{precs[0]}, dx=0.25, dt=dx/10, {a.ic} initial state",
-                   "lattice": n, "precision": precs[0], "variant": a.variant, "ic": a.ic},
+                   "lattice": n, "precision": precs[0], "variant": head["variant"], "ic": a.ic},
         "roofline": head["roofline"],
         "step_roofline_frac": head["step_roofline_frac"],
         "stage_ms": head["stage_ms"],
@@ -427,7 +462,7 @@ def main(argv=None) -> int:
     }
     if "mixed" in res and precs[0] != "mixed":
         m = res["mixed"]
-        out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"],
+        out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"], "variant": m["variant"],
                         "speedup_vs_fp64": m["value"] / head["value"], "dtype": "f32 storage/stencils, f64 reductions",
                         "roofline": m["roofline"], "step_roofline_frac": m["step_roofline_frac"],
                         "stage_ms": m["stage_ms"], "energy_check": m["energy_check"],

@@ -694,6 +694,13 @@ int ewpt_get_variant(ewpt_ctx *ctx, int *variant)
     return EWPT_OK;
 }
 
+int ewpt_active_variant(ewpt_ctx *ctx, int *variant)
+{
+    if (!ctx || !variant) return EWPT_ERR_INVAL;
+    *variant = resolved_variant(ctx);
+    return EWPT_OK;
+}
+
 int ewpt_device_bytes(ewpt_ctx *ctx, unsigned long long *bytes)
 {
     if (!ctx || !bytes) return EWPT_ERR_INVAL;

@@ -201,6 +201,15 @@ __device__ __forceinline__ void sto(T *base, unsigned boff, T v)
 {
     *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
 }
+// Streaming (non-temporal) store: no launch re-reads what it writes, so
+// its lines are evicted first and leave L2 to the halo lines that
+// neighbouring bricks re-read (streaming loads of the centre-only values
+// measured slower).
+template <typename T>
+__device__ __forceinline__ void stn(T *base, unsigned boff, T v)
+{
+    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff));
+}
 
 // Workgroup barrier for LDS only.  __syncthreads() also fences global
 // memory, which makes every in-flight global load (the prefetched halo of
@@ -324,13 +333,13 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 dy = k4 * d6 + dy;
                 R nh, nl;
                 two_sum(ldo(Y(ch), qb), ldo(b.ylo + (long)ch * b.n3, qb) + dy, nh, nl);
-                sto(b.ynew + (long)ch * b.n3, qb, nh);
-                sto(b.ylo_new + (long)ch * b.n3, qb, nl);
+                stn(b.ynew + (long)ch * b.n3, qb, nh);
+                stn(b.ylo_new + (long)ch * b.n3, qb, nl);
             } else {
                 R n = piv * d6 + ldo(Y(ch), qb);
                 n = k2 * d3 + n;
                 n = k3 * d3 + n;
-                sto(b.ynew + (long)ch * b.n3, qb, k4 * d6 + n);
+                stn(b.ynew + (long)ch * b.n3, qb, k4 * d6 + n);
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -686,7 +695,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 
     if constexpr (S < 4) {
 #pragma unroll
-        for (int m = 0; m < NK; m++) sto(b.kout + (long)m * b.n3, qb, o[m]);
+        for (int m = 0; m < NK; m++) stn(b.kout + (long)m * b.n3, qb, o[m]);
         if constexpr (stores_next(PF, S)) {
             // next stage input: fields y + c * (this stage's momentum input),
             // the other channels y + c * k_s (the expressions of stage_in)
@@ -695,7 +704,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             for (int ch = 0; ch < NCH; ch++) {
                 const R yv = S == 1 ? c[ch] : R(ldo(Y(ch), qb));
                 const R kv = ch < 16 ? c[ch + 20] : o[ch - 16];
-                sto(b.unext + (long)ch * b.n3, qb, yv + kv * cc);
+                stn(b.unext + (long)ch * b.n3, qb, yv + kv * cc);
             }
         }
     } else if constexpr (MIXED) {
@@ -703,12 +712,12 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         for (int m = 0; m < NK; m++) {
             R nh, nl;
             two_sum(nstash[(NK + m) * B::NT + t], o[m] * d6 + nstash[m * B::NT + t], nh, nl);
-            sto(b.ynew + (long)(16 + m) * b.n3, qb, nh);
-            sto(b.ylo_new + (long)(16 + m) * b.n3, qb, nl);
+            stn(b.ynew + (long)(16 + m) * b.n3, qb, nh);
+            stn(b.ylo_new + (long)(16 + m) * b.n3, qb, nl);
         }
     } else {
 #pragma unroll
-        for (int m = 0; m < NK; m++) sto(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
+        for (int m = 0; m < NK; m++) stn(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
     }
 }
 

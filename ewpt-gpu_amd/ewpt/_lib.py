@@ -30,7 +30,7 @@ EWPT_MIXED = 1
 EXPORTS = (
     "ewpt_create", "ewpt_destroy", "ewpt_set_state", "ewpt_get_state", "ewpt_rhs",
     "ewpt_rhs_device", "ewpt_step", "ewpt_step_timed", "ewpt_energies", "ewpt_gauss",
-    "ewpt_set_variant", "ewpt_get_variant", "ewpt_device_bytes", "ewpt_strerror",
+    "ewpt_set_variant", "ewpt_get_variant", "ewpt_active_variant", "ewpt_device_bytes", "ewpt_strerror",
     "ewpt_last_error", "ewpt_version", "ewpt_init_modes", "ewpt_init_bubbles", "ewpt_init_thermal",
     "ewpt_em_energies", "ewpt_em_fields", "ewpt_spectrum", "ewpt_y_slice",
     "ewpt_batch_create", "ewpt_batch_destroy", "ewpt_batch_last_error", "ewpt_batch_set_variant",
@@ -88,6 +88,7 @@ def load() -> ctypes.CDLL:
         "ewpt_gauss": [vp, pp, dp],
         "ewpt_set_variant": [vp, ip],
         "ewpt_get_variant": [vp, ctypes.POINTER(ip)],
+        "ewpt_active_variant": [vp, ctypes.POINTER(ip)],
         "ewpt_device_bytes": [vp, ctypes.POINTER(ctypes.c_ulonglong)],
         "ewpt_init_modes": [vp, ctypes.c_ulonglong, ip, ip, ctypes.c_double, ctypes.c_double,
                             ctypes.c_double, dp],

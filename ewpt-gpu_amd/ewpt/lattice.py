@@ -215,6 +215,13 @@ class Lattice:
     def variant(self, v: int):
         self._check(self._L.ewpt_set_variant(self._h, int(v)), "ewpt_set_variant")
 
+    @property
+    def active_variant(self) -> int:
+        """The stage variant the next step launches (what 0 resolves to)."""
+        v = ctypes.c_int()
+        self._check(self._L.ewpt_active_variant(self._h, ctypes.byref(v)), "ewpt_active_variant")
+        return v.value
+
     def device_bytes(self) -> int:
         b = ctypes.c_ulonglong()
         self._check(self._L.ewpt_device_bytes(self._h, ctypes.byref(b)), "ewpt_device_bytes")

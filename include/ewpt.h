@@ -222,6 +222,8 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  * EWPT_ERR_UNSUPPORTED. */
 int ewpt_set_variant(ewpt_ctx *ctx, int variant);
 int ewpt_get_variant(ewpt_ctx *ctx, int *variant);
+/* The variant the next step launches (what 0 = automatic resolves to). */
+int ewpt_active_variant(ewpt_ctx *ctx, int *variant);
 
 /* Device bytes the context holds. */
 int ewpt_device_bytes(ewpt_ctx *ctx, unsigned long long *bytes);
