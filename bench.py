@@ -278,6 +278,64 @@ def corrector_bench(a) -> int:
     return 0
 
 
+XGMI_LINK = 153e9   # bytes/s of one MI355X xGMI link (one direction)
+
+
+def slab_bench(a) -> int:
+    """One GPU's share of the 512^3 lattice split into --shards z-slabs: the
+    stage kernels on the 512 x 512 x (512/shards) slab reading their z halo
+    from ghost planes, plus the halo pack/unpack of every stage (here the
+    slab is its own neighbour, so the exchange is two device copies)."""
+    import ewpt
+    from ewpt import _lib
+
+    n, dx, T = a.n, 0.25, 0.5
+    if n % a.shards:
+        print(f"bench.py: --shards {a.shards} does not divide {n}", file=sys.stderr)
+        return 2
+    local = (n, n, n // a.shards)
+    params = ewpt.Parameters(N=n, dx=dx, gy=0.35, T=T)
+    P = params.step_params()
+    sites = n * n * (n // a.shards)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        prec = _lib.EWPT_FP64 if precision == "fp64" else _lib.EWPT_MIXED
+        print(f"bench: slab {local} {precision} warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+        with ewpt.Lattice(local, dx, precision=prec) as src:
+            src.init_thermal(T, params.meff_sq, params.base_seed, "config")
+            with ewpt.Lattice(local, dx, precision=prec, slab=True) as slab:
+                slab.copy_from(src)
+                ewpt.slab_step([slab], P, a.warmup)
+                stage_ms, exch_ms, total_ms = ewpt.slab_step_timed([slab], P, a.steps)
+                variant = slab.active_variant
+                halo = sum(slab.halo_bytes(w) for w in range(4))    # per side per step
+            # the same lattice as a periodic context (z wrapped, no exchange)
+            src.variant = variant
+            if a.warmup:
+                src.step(P, a.warmup)
+            p_stage, p_total = src.step_timed(P, a.steps)
+        out[precision] = {
+            "value": sites * a.steps / (total_ms * 1e-3), "ms_per_step": total_ms / a.steps, "variant": variant,
+            "stage_ms": stage_ms, "exchange_ms_per_step": exch_ms,
+            "periodic_ms_per_step": p_total / a.steps, "periodic_stage_ms": p_stage,
+            "halo_bytes_per_side_per_step": halo,
+            "xgmi_ms_per_step_estimate": halo / XGMI_LINK * 1e3,
+        }
+    f = out["fp64"]
+    print(json.dumps({
+        "metric": f"site-updates/s of one GPU's share of {n}^3 over {a.shards} GPUs (z-slab with halo exchange)",
+        "value": f["value"], "unit": "site-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": f["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: seeded thermal IC (T=0.5) of the slab's own periodic lattice",
+        "config": {"workload": f"RK4 steps of a {local[0]}x{local[1]}x{local[2]} z-slab with 3 ghost planes, "
+                               "halo pack/unpack before every stage (the slab is its own z-neighbour; the "
+                               "transfer between GPUs is not measured: xgmi_ms_per_step_estimate is bytes / "
+                               "one link's 153 GB/s)", "slab": list(local), "shards": a.shards},
+        "fp64": f, "mixed": out["mixed"],
+    }))
+    return 0
+
+
 def precision_study(a) -> int:
     """Mixed path against FP64 from the same thermal IC over a long run: per
     field-channel relative L2 error, energy drift of each, Gauss-norm ratio.
@@ -362,6 +420,9 @@ def main(argv=None) -> int:
     ap.add_argument("--precision-study", action="store_true",
                     help="mixed vs FP64 over --study-steps steps from the same thermal IC (errors, drifts)")
     ap.add_argument("--study-steps", type=int, default=1000)
+    ap.add_argument("--slab", action="store_true",
+                    help="one GPU's share of the --n lattice split into --shards z-slabs, with halo exchange")
+    ap.add_argument("--shards", type=int, default=8)
     ap.add_argument("--stream-copy", action="store_true",
                     help="only time a plain device copy (HBM ceiling / counter calibration)")
     a = ap.parse_args(argv)
@@ -371,6 +432,12 @@ def main(argv=None) -> int:
 
     if a.precision_study:
         return precision_study(a)
+
+    if a.slab:
+        if a.gpus != 1:
+            print("bench.py: the slab configuration measures one GPU's share on one GPU", file=sys.stderr)
+            return 2
+        return slab_bench(a)
 
     if a.corrector:
         if a.gpus != 1:

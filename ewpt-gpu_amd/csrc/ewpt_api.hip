@@ -23,8 +23,15 @@ struct ewpt_ctx {
     int precision = EWPT_FP64;
     int variant = 0;
     ewpt_lattice lat{};
-    long n3 = 0;
+    long n3 = 0;            // channel stride in elements (sites, plus ghost planes for a slab)
     hipStream_t stream = nullptr;
+    // z-slab contexts (ewpt_create_slab): HALO ghost planes below and above
+    // the interior in every array, the send buffers of the halo exchange and
+    // the events that order it between contexts
+    int zg = 0;
+    long plane = 0;         // nx * ny
+    void *halo_lo = nullptr, *halo_hi = nullptr;
+    hipEvent_t ev_packed = nullptr, ev_unpacked = nullptr;
     // FP64 path: state ping-pong and the three stored stage derivatives.
     // Mixed path: y is an FP64 view of the state (for state I/O and the
     // FP64 reductions), rebuilt from the FP32 pair when view_valid is false.
@@ -94,6 +101,11 @@ void release(ewpt_ctx *c)
     float **fbufs[] = {&c->fhi, &c->flo, &c->fhi_n, &c->flo_n, &c->fk[0], &c->fk[1], &c->fk[2], &c->fu};
     for (float **b : fbufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
+    void **hbufs[] = {&c->halo_lo, &c->halo_hi};
+    for (void **b : hbufs)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
+    if (c->ev_packed) { (void)hipEventDestroy(c->ev_packed); c->ev_packed = nullptr; }
+    if (c->ev_unpacked) { (void)hipEventDestroy(c->ev_unpacked); c->ev_unpacked = nullptr; }
     if (c->stream) { (void)hipStreamDestroy(c->stream); c->stream = nullptr; }
 }
 
@@ -132,6 +144,7 @@ StageBufs<double> bufs_for(ewpt_ctx *c, int stage, bool ustaged = false)
     if (stage < 4) b.kout = c->k[stage - 1];
     else b.ynew = c->ynext;
     if (ustaged) set_stage_inputs(b, stage, c->u, c->ynext);
+    b.zghost = c->zg;
     return b;
 }
 
@@ -144,6 +157,31 @@ int ensure_stage_input_buffer(ewpt_ctx *c, int variant)
 }
 
 int set_device(ewpt_ctx *c) { HIPCHK(c, hipSetDevice(c->device), "hipSetDevice"); return EWPT_OK; }
+
+// sites of the lattice (a slab's interior); equals n3 without ghost planes
+long sites(const ewpt_ctx *c) { return c->plane * c->lat.nz; }
+
+// copy nch channels of `count` elements between arrays of different channel
+// strides (a slab's interior and a dense array)
+hipError_t copy_channels(void *dst, long dstride, long doff, const void *src, long sstride, long soff, int nch,
+                         long count, size_t elem, hipMemcpyKind kind, hipStream_t st)
+{
+    if (dstride == count && sstride == count && doff == 0 && soff == 0)
+        return hipMemcpyAsync(dst, src, nch * count * elem, kind, st);
+    for (int ch = 0; ch < nch; ch++) {
+        hipError_t e = hipMemcpyAsync(static_cast<char *>(dst) + (ch * dstride + doff) * elem,
+                                      static_cast<const char *>(src) + (ch * sstride + soff) * elem, count * elem,
+                                      kind, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// entry points that need the whole periodic lattice on one context
+int no_slab(ewpt_ctx *c, const char *what)
+{
+    return fail(c, EWPT_ERR_UNSUPPORTED, (std::string(what) + ": not available on a slab context").c_str());
+}
 
 bool mixed(const ewpt_ctx *c) { return c->precision == EWPT_MIXED; }
 
@@ -184,7 +222,7 @@ int resolved_variant(const ewpt_ctx *c)
     return VARIANT_GATHER;
 }
 
-int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
+int launch_stage_kernel(ewpt_ctx *c, int stage, const ewpt_params *p)
 {
     const Phys P = phys_of(*p);
     const int v = resolved_variant(c);
@@ -204,6 +242,7 @@ int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
         if (stage < 4) b.kout = c->fk[stage - 1];
         else { b.ynew = c->fhi_n; b.ylo_new = c->flo_n; }
         if (us) set_stage_inputs(b, stage, c->fu, c->fhi_n);
+        b.zghost = c->zg;
         HIPCHK(c, launch_stage_brick_f32(v, stage, b, c->lat, P, p->dt, c->stream), "stage launch");
         if (stage == 4) {
             std::swap(c->fhi, c->fhi_n);
@@ -218,6 +257,89 @@ int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
         HIPCHK(c, launch_stage_brick(v, stage, bufs_for(c, stage, us), c->lat, P, p->dt, c->stream), "stage launch");
     if (stage == 4) std::swap(c->y, c->ynext);
     return EWPT_OK;
+}
+
+// ---- halo exchange of z-slab contexts
+// what = 0: the state (mixed: its FP32 high word, all the stage inputs read);
+// what = s in 1..3: the stage derivative k_s.  Stage s reads at halo sites
+// the state and k_{s-1} (ewpt_stage.h), so the state is exchanged before
+// stage 1 and k_s after stage s.
+void *halo_array(ewpt_ctx *c, int what, int &nch)
+{
+    nch = what == 0 ? NCH : NK;
+    if (mixed(c)) return what == 0 ? static_cast<void *>(c->fhi) : static_cast<void *>(c->fk[what - 1]);
+    return what == 0 ? static_cast<void *>(c->y) : static_cast<void *>(c->k[what - 1]);
+}
+
+size_t halo_bytes(const ewpt_ctx *c, int what)
+{
+    return (size_t)(what == 0 ? NCH : NK) * HALO * c->plane * (mixed(c) ? sizeof(float) : sizeof(double));
+}
+
+int halo_pack_into(ewpt_ctx *c, int what, void *lo, void *hi)
+{
+    int nch;
+    void *a = halo_array(c, what, nch);
+    hipError_t e = mixed(c) ? launch_halo_pack(static_cast<const float *>(a), c->n3, c->plane, c->lat.nz, nch,
+                                               static_cast<float *>(lo), static_cast<float *>(hi), c->stream)
+                            : launch_halo_pack(static_cast<const double *>(a), c->n3, c->plane, c->lat.nz, nch,
+                                               static_cast<double *>(lo), static_cast<double *>(hi), c->stream);
+    HIPCHK(c, e, "halo pack");
+    return EWPT_OK;
+}
+
+int halo_unpack_from(ewpt_ctx *c, int what, const void *lo, const void *hi)
+{
+    int nch;
+    void *a = halo_array(c, what, nch);
+    hipError_t e = mixed(c) ? launch_halo_unpack(static_cast<float *>(a), c->n3, c->plane, c->lat.nz, nch,
+                                                 static_cast<const float *>(lo), static_cast<const float *>(hi),
+                                                 c->stream)
+                            : launch_halo_unpack(static_cast<double *>(a), c->n3, c->plane, c->lat.nz, nch,
+                                                 static_cast<const double *>(lo), static_cast<const double *>(hi),
+                                                 c->stream);
+    HIPCHK(c, e, "halo unpack");
+    return EWPT_OK;
+}
+
+// One exchange over a ring of slabs on one device (slab i's z-neighbours
+// are i-1 and i+1 mod n; n = 1 is periodic in z): every slab packs into its
+// own send buffers, then unpacks its neighbours'.  Events order the streams:
+// an unpack waits for the neighbours' packs, and a pack waits for the
+// neighbours' unpacks of the previous exchange, which read its buffers.
+int ring_exchange(ewpt_ctx *const *sl, int n, int what)
+{
+    for (int i = 0; i < n; i++) {
+        ewpt_ctx *c = sl[i], *lo = sl[(i + n - 1) % n], *hi = sl[(i + 1) % n];
+        if (n > 1) {
+            HIPCHK(c, hipStreamWaitEvent(c->stream, lo->ev_unpacked, 0), "wait unpack");
+            HIPCHK(c, hipStreamWaitEvent(c->stream, hi->ev_unpacked, 0), "wait unpack");
+        }
+        if (int rc = halo_pack_into(c, what, c->halo_lo, c->halo_hi)) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_packed, c->stream), "record pack");
+    }
+    for (int i = 0; i < n; i++) {
+        ewpt_ctx *c = sl[i], *lo = sl[(i + n - 1) % n], *hi = sl[(i + 1) % n];
+        if (n > 1) {
+            HIPCHK(c, hipStreamWaitEvent(c->stream, lo->ev_packed, 0), "wait pack");
+            HIPCHK(c, hipStreamWaitEvent(c->stream, hi->ev_packed, 0), "wait pack");
+        }
+        // the ghost planes below come from the lower neighbour's top planes
+        if (int rc = halo_unpack_from(c, what, lo->halo_hi, hi->halo_lo)) return rc;
+        HIPCHK(c, hipEventRecord(c->ev_unpacked, c->stream), "record unpack");
+    }
+    return EWPT_OK;
+}
+
+// One stage of a step; a slab context on its own is periodic in z and
+// exchanges with itself first.
+int launch_step_stage(ewpt_ctx *c, int stage, const ewpt_params *p)
+{
+    if (c->zg) {
+        ewpt_ctx *self[1] = {c};
+        if (int rc = ring_exchange(self, 1, stage - 1)) return rc;
+    }
+    return launch_stage_kernel(c, stage, p);
 }
 
 }  // namespace
@@ -242,12 +364,17 @@ const char *ewpt_strerror(int code)
 
 const char *ewpt_last_error(ewpt_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
 
-int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int device)
+namespace {
+
+// zg = 0: periodic lattice; zg = HALO: z-slab with ghost planes
+int create_ctx(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int device, int zg)
 {
     if (!out) return EWPT_ERR_INVAL;
     *out = nullptr;
     if (!lattice_ok(lat) || (precision != EWPT_FP64 && precision != EWPT_MIXED) || device < 0)
         return EWPT_ERR_INVAL;
+    // stage kernels address a channel with 32-bit byte offsets
+    if (zg && (long)lat->nx * lat->ny * (lat->nz + 2L * zg) * 8 >= (1L << 31)) return EWPT_ERR_UNSUPPORTED;
     if (precision == EWPT_MIXED && !brick_supported_f32(210, *lat) && !brick_supported_f32(211, *lat))
         return EWPT_ERR_UNSUPPORTED;   // the mixed path has LDS-brick kernels only
     int ndev = 0;
@@ -260,7 +387,10 @@ int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devi
     c->device = device;
     c->precision = precision;
     c->lat = *lat;
-    c->n3 = (long)lat->nx * lat->ny * lat->nz;
+    c->zg = zg;
+    c->plane = (long)lat->nx * lat->ny;
+    c->n3 = c->plane * (lat->nz + 2L * zg);
+    if (zg) c->stored_ok = false;   // the exchange carries y and the k's, not stored stage inputs
     int rc = set_device(c);
     if (rc == EWPT_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(c, EWPT_ERR_HIP, "hipStreamCreate");
@@ -278,6 +408,18 @@ int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devi
     c->nred = reduce_blocks(c->lat);
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->part, (size_t)c->nred * NRED_SLOTS, "alloc partials");
     if (rc == EWPT_OK) rc = dev_alloc(c, &c->res, NRED_SLOTS, "alloc results");
+    if (rc == EWPT_OK && zg) {
+        const int v = resolved_variant(c);
+        if (v < 0 || v == VARIANT_GATHER) rc = fail(c, EWPT_ERR_UNSUPPORTED, "no brick variant tiles this slab");
+        const size_t hb = (size_t)NCH * HALO * c->plane * (precision == EWPT_MIXED ? sizeof(float) : sizeof(double));
+        unsigned char **hp[] = {reinterpret_cast<unsigned char **>(&c->halo_lo),
+                                reinterpret_cast<unsigned char **>(&c->halo_hi)};
+        for (unsigned char **b : hp)
+            if (rc == EWPT_OK) rc = dev_alloc(c, b, hb, "alloc halo buffers");
+        if (rc == EWPT_OK && (hipEventCreateWithFlags(&c->ev_packed, hipEventDisableTiming) != hipSuccess ||
+                              hipEventCreateWithFlags(&c->ev_unpacked, hipEventDisableTiming) != hipSuccess))
+            rc = fail(c, EWPT_ERR_HIP, "hipEventCreate");
+    }
     if (rc == EWPT_OK && brick_stored_inputs(resolved_variant(c))) {
         // AUTO's first choice wants the stage-input buffer: take it now, or
         // fall back to the variants that do without
@@ -297,6 +439,18 @@ int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devi
     return EWPT_OK;
 }
 
+}  // namespace
+
+int ewpt_create(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int device)
+{
+    return create_ctx(out, lat, precision, device, 0);
+}
+
+int ewpt_create_slab(ewpt_ctx **out, const ewpt_lattice *local, int precision, int device)
+{
+    return create_ctx(out, local, precision, device, HALO);
+}
+
 int ewpt_destroy(ewpt_ctx *ctx)
 {
     if (!ctx) return EWPT_ERR_INVAL;
@@ -312,7 +466,8 @@ int ewpt_set_state(ewpt_ctx *ctx, const double *host)
     if (!ctx || !host) return EWPT_ERR_INVAL;
     int rc = set_device(ctx);
     if (rc) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->y, host, NCH * ctx->n3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+    HIPCHK(ctx, copy_channels(ctx->y, ctx->n3, ctx->zg * ctx->plane, host, sites(ctx), 0, NCH, sites(ctx),
+                              sizeof(double), hipMemcpyHostToDevice, ctx->stream),
            "set_state copy");
     if ((rc = push_view(ctx))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "set_state sync");
@@ -327,7 +482,8 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host)
     int rc = set_device(ctx);
     if (rc) return rc;
     if ((rc = ensure_view(ctx))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(host, ctx->y, NCH * ctx->n3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+    HIPCHK(ctx, copy_channels(host, sites(ctx), 0, ctx->y, ctx->n3, ctx->zg * ctx->plane, NCH, sites(ctx),
+                              sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
            "get_state copy");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "get_state sync");
     return EWPT_OK;
@@ -362,13 +518,18 @@ int ewpt_copy_state(ewpt_ctx *dst, ewpt_ctx *src)
     if (!src->has_state) return fail(dst, EWPT_ERR_STATE, "copy_state: source has no state");
     int rc = set_device(src);
     if (rc) return rc;
-    const size_t n = NCH * (size_t)src->n3;
+    // interior only, so slab and periodic contexts of one shape copy both ways
+    const long n = sites(src), so = src->zg * src->plane, d0 = dst->zg * dst->plane;
+    const hipMemcpyKind dd = hipMemcpyDeviceToDevice;
     if (mixed(src)) {
-        HIPCHK(dst, hipMemcpyAsync(dst->fhi, src->fhi, n * sizeof(float), hipMemcpyDeviceToDevice, src->stream), "copy hi");
-        HIPCHK(dst, hipMemcpyAsync(dst->flo, src->flo, n * sizeof(float), hipMemcpyDeviceToDevice, src->stream), "copy lo");
+        HIPCHK(dst, copy_channels(dst->fhi, dst->n3, d0, src->fhi, src->n3, so, NCH, n, sizeof(float), dd, src->stream),
+               "copy hi");
+        HIPCHK(dst, copy_channels(dst->flo, dst->n3, d0, src->flo, src->n3, so, NCH, n, sizeof(float), dd, src->stream),
+               "copy lo");
         dst->view_valid = false;
     } else {
-        HIPCHK(dst, hipMemcpyAsync(dst->y, src->y, n * sizeof(double), hipMemcpyDeviceToDevice, src->stream), "copy y");
+        HIPCHK(dst, copy_channels(dst->y, dst->n3, d0, src->y, src->n3, so, NCH, n, sizeof(double), dd, src->stream),
+               "copy y");
     }
     HIPCHK(dst, hipStreamSynchronize(src->stream), "copy sync");
     dst->has_state = true;
@@ -381,6 +542,7 @@ int ewpt_init_modes(ewpt_ctx *ctx, unsigned long long seed, int nmodes, int kmax
     if (!ctx || nmodes < 1 || nmodes > 64 || kmax < 0 || !std::isfinite(amp) || !std::isfinite(vev) ||
         !std::isfinite(mom_scale))
         return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "init_modes");
     int rc = set_device(ctx);
     if (rc) return rc;
     const size_t nt = (size_t)NCH * nmodes * 5;
@@ -404,6 +566,7 @@ int ewpt_init_bubbles(ewpt_ctx *ctx, const double *bubbles, int nb, int profile,
     if (!ctx || nb < 0 || nb > (1 << 20) || (nb > 0 && !bubbles) || (profile != 0 && profile != 1) ||
         !std::isfinite(mH) || !(mH > 0.0))
         return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "init_bubbles");
     for (long q = 0; q < 7L * nb; q++)
         if (!std::isfinite(bubbles[q])) return fail(ctx, EWPT_ERR_INVAL, "init_bubbles: non-finite bubble table");
     int rc = set_device(ctx);
@@ -427,6 +590,7 @@ int ewpt_init_thermal(ewpt_ctx *ctx, double T, double meff_sq, unsigned long lon
 {
     if (!ctx || !std::isfinite(T) || !(T > 0.0) || !std::isfinite(meff_sq) || (profile != 0 && profile != 1))
         return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "init_thermal");
     int rc = set_device(ctx);
     if (rc) return rc;
     // the stored stage derivatives are free between steps: they hold the
@@ -455,6 +619,7 @@ int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p, const double 
 int ewpt_rhs(ewpt_ctx *ctx, const ewpt_params *p, const double *hin, double *hout)
 {
     if (!ctx || !hin || !hout || !params_ok(p)) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "rhs");
     int rc = set_device(ctx);
     if (rc) return rc;
     const size_t n = NCH * (size_t)ctx->n3;
@@ -518,9 +683,163 @@ int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double stag
     return rc;
 }
 
+// ---------------------------------------------------------------- z-slabs
+
+namespace {
+
+int slab_ok(ewpt_ctx *c, int what)
+{
+    if (!c->zg) return fail(c, EWPT_ERR_INVAL, "not a slab context");
+    if (what < 0 || what > 3) return fail(c, EWPT_ERR_INVAL, "halo array must be 0 (state) or 1..3 (k_s)");
+    if (!c->has_state) return fail(c, EWPT_ERR_STATE, "halo exchange before set_state");
+    return EWPT_OK;
+}
+
+int ring_ok(ewpt_ctx *const *sl, int n)
+{
+    if (!sl || n < 1) return EWPT_ERR_INVAL;
+    for (int i = 0; i < n; i++) {
+        if (!sl[i]) return EWPT_ERR_INVAL;
+        for (int j = 0; j < i; j++)
+            if (sl[j] == sl[i]) return fail(sl[0], EWPT_ERR_INVAL, "slab_step: a context appears twice");
+        if (!sl[i]->zg) return fail(sl[0], EWPT_ERR_INVAL, "slab_step: not a slab context");
+        if (sl[i]->device != sl[0]->device || sl[i]->precision != sl[0]->precision ||
+            sl[i]->lat.nx != sl[0]->lat.nx || sl[i]->lat.ny != sl[0]->lat.ny || sl[i]->lat.dx != sl[0]->lat.dx)
+            return fail(sl[0], EWPT_ERR_INVAL, "slab_step: slabs differ in device, precision, nx, ny or dx");
+        if (!sl[i]->has_state) return fail(sl[0], EWPT_ERR_STATE, "slab_step before set_state");
+    }
+    return EWPT_OK;
+}
+
+// nsteps steps of a ring of slabs; with timing, events on slab 0's stream
+// split each stage into its exchange and its kernel
+int ring_steps(ewpt_ctx *const *sl, int n, const ewpt_params *p, int nsteps, double *stage_ms, double *exch_ms,
+               double *total_ms)
+{
+    ewpt_ctx *c0 = sl[0];
+    std::vector<hipEvent_t> ev;
+    int rc = EWPT_OK;
+    auto mark = [&]() {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return fail(c0, EWPT_ERR_HIP, "hipEventCreate");
+        ev.push_back(e);
+        if (hipEventRecord(e, c0->stream) != hipSuccess) return fail(c0, EWPT_ERR_HIP, "event record");
+        return EWPT_OK;
+    };
+    if (stage_ms) rc = mark();
+    for (int it = 0; it < nsteps && rc == EWPT_OK; it++)
+        for (int s = 1; s <= 4 && rc == EWPT_OK; s++) {
+            rc = ring_exchange(sl, n, s - 1);
+            if (rc == EWPT_OK && stage_ms) rc = mark();
+            for (int i = 0; i < n && rc == EWPT_OK; i++) rc = launch_stage_kernel(sl[i], s, p);
+            if (rc == EWPT_OK && stage_ms) rc = mark();
+        }
+    for (int i = 0; i < n; i++)
+        if (hipStreamSynchronize(sl[i]->stream) != hipSuccess && rc == EWPT_OK) rc = fail(sl[i], EWPT_ERR_HIP, "sync");
+    if (rc == EWPT_OK && stage_ms) {
+        for (int s = 0; s < 4; s++) stage_ms[s] = 0.0;
+        *exch_ms = 0.0;
+        for (int it = 0; it < nsteps; it++)
+            for (int s = 0; s < 4; s++) {
+                const size_t k = 1 + 2 * (4 * (size_t)it + s);
+                float a = 0.f, b = 0.f;
+                (void)hipEventElapsedTime(&a, ev[k - 1], ev[k]);
+                (void)hipEventElapsedTime(&b, ev[k], ev[k + 1]);
+                *exch_ms += a;
+                stage_ms[s] += b;
+            }
+        for (int s = 0; s < 4; s++) stage_ms[s] /= nsteps;
+        *exch_ms /= nsteps;
+        float tot = 0.f;
+        (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+        *total_ms = tot;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+}  // namespace
+
+int ewpt_slab_halo_bytes(ewpt_ctx *ctx, int what, unsigned long long *bytes)
+{
+    if (!ctx || !bytes) return EWPT_ERR_INVAL;
+    if (!ctx->zg) return fail(ctx, EWPT_ERR_INVAL, "not a slab context");
+    if (what < 0 || what > 3) return EWPT_ERR_INVAL;
+    *bytes = halo_bytes(ctx, what);
+    return EWPT_OK;
+}
+
+int ewpt_slab_buffers(ewpt_ctx *ctx, void **send_lo, void **send_hi)
+{
+    if (!ctx || !send_lo || !send_hi) return EWPT_ERR_INVAL;
+    if (!ctx->zg) return fail(ctx, EWPT_ERR_INVAL, "not a slab context");
+    *send_lo = ctx->halo_lo;
+    *send_hi = ctx->halo_hi;
+    return EWPT_OK;
+}
+
+int ewpt_slab_pack(ewpt_ctx *ctx, int what, void *send_lo, void *send_hi)
+{
+    if (!ctx || !send_lo || !send_hi) return EWPT_ERR_INVAL;
+    int rc = slab_ok(ctx, what);
+    if (rc || (rc = set_device(ctx))) return rc;
+    return halo_pack_into(ctx, what, send_lo, send_hi);
+}
+
+int ewpt_slab_unpack(ewpt_ctx *ctx, int what, const void *recv_lo, const void *recv_hi)
+{
+    if (!ctx || !recv_lo || !recv_hi) return EWPT_ERR_INVAL;
+    int rc = slab_ok(ctx, what);
+    if (rc || (rc = set_device(ctx))) return rc;
+    return halo_unpack_from(ctx, what, recv_lo, recv_hi);
+}
+
+int ewpt_stage(ewpt_ctx *ctx, const ewpt_params *p, int stage)
+{
+    if (!ctx || !params_ok(p) || stage < 1 || stage > 4) return EWPT_ERR_INVAL;
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "stage before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    return launch_stage_kernel(ctx, stage, p);
+}
+
+int ewpt_get_stream(ewpt_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return EWPT_ERR_INVAL;
+    *stream = static_cast<void *>(ctx->stream);
+    return EWPT_OK;
+}
+
+int ewpt_sync(ewpt_ctx *ctx)
+{
+    if (!ctx) return EWPT_ERR_INVAL;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return EWPT_OK;
+}
+
+int ewpt_slab_step(ewpt_ctx *const *slabs, int n, const ewpt_params *p, int nsteps)
+{
+    if (!params_ok(p) || nsteps < 0) return EWPT_ERR_INVAL;
+    int rc = ring_ok(slabs, n);
+    if (rc || (rc = set_device(slabs[0]))) return rc;
+    return ring_steps(slabs, n, p, nsteps, nullptr, nullptr, nullptr);
+}
+
+int ewpt_slab_step_timed(ewpt_ctx *const *slabs, int n, const ewpt_params *p, int nsteps, double stage_ms[4],
+                         double *exchange_ms, double *total_ms)
+{
+    if (!params_ok(p) || nsteps <= 0 || !stage_ms || !exchange_ms || !total_ms) return EWPT_ERR_INVAL;
+    int rc = ring_ok(slabs, n);
+    if (rc || (rc = set_device(slabs[0]))) return rc;
+    return ring_steps(slabs, n, p, nsteps, stage_ms, exchange_ms, total_ms);
+}
+
 int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13])
 {
     if (!ctx || !out || !params_ok(p)) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "energies");
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "energies before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
@@ -557,6 +876,7 @@ bool theta_ok(double th) { return std::isfinite(th); }
 int ewpt_em_energies(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double out[3])
 {
     if (!ctx || !out || !params_ok(p) || !theta_ok(theta_w)) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "em_energies");
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "em_energies before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
@@ -577,6 +897,7 @@ int ewpt_em_energies(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double
 int ewpt_em_fields(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, double *host_out)
 {
     if (!ctx || !host_out || !params_ok(p) || !theta_ok(theta_w)) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "em_fields");
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "em_fields before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
@@ -593,6 +914,7 @@ int ewpt_y_slice(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int j, dou
 {
     if (!ctx || !host_out || !params_ok(p) || !theta_ok(theta_w) || j < 0 || j >= ctx->lat.ny)
         return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "y_slice");
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "y_slice before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
@@ -611,6 +933,7 @@ int ewpt_spectrum(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int field
     if (!ctx || !params_ok(p) || !theta_ok(theta_w) || (field != 0 && field != 1) || !cut || !nbins ||
         (!k_out) != (!p_out))
         return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "spectrum");
     for (int d = 0; d < 3; d++)
         if (cut[d] < 1 || cut[d] > 4096) return fail(ctx, EWPT_ERR_INVAL, "spectrum: cut out of range");
     const char *what = "";
@@ -641,6 +964,7 @@ int ewpt_spectrum(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int field
 int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8])
 {
     if (!ctx || !out || !params_ok(p)) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "gauss");
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "gauss before set_state");
     int rc = set_device(ctx);
     if (rc) return rc;
@@ -666,6 +990,8 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
         ctx->variant = variant;
         return EWPT_OK;
     }
+    if (ctx->zg && (variant == VARIANT_GATHER || brick_stored_inputs(variant)))
+        return fail(ctx, EWPT_ERR_UNSUPPORTED, "slab contexts take the brick variants without stored stage inputs");
     if (mixed(ctx)) {
         if (variant % 100 < 10 || variant % 100 > 19) return EWPT_ERR_INVAL;
         if (!brick_exists(variant)) return EWPT_ERR_INVAL;
@@ -1058,6 +1384,7 @@ static int net_ctx_ok(ewpt_net *n, ewpt_ctx *c)
     if (c->device != n->device) return nfail(n, EWPT_ERR_INVAL, "net: context on another device");
     if (c->precision != EWPT_FP64) return nfail(n, EWPT_ERR_UNSUPPORTED, "net: FP64 contexts only");
     if (!net_lattice_ok(c->lat)) return nfail(n, EWPT_ERR_UNSUPPORTED, "net: lattice must tile 8x4x4");
+    if (c->zg) return nfail(n, EWPT_ERR_UNSUPPORTED, "net: not available on a slab context");
     if (!c->has_state) return nfail(n, EWPT_ERR_STATE, "net: context has no state");
     return EWPT_OK;
 }
@@ -1162,6 +1489,7 @@ int ewpt_batch_create(ewpt_batch **out, ewpt_ctx *const *ctxs, int n)
     for (int i = 0; i < n; i++) {
         if (!ctxs[i]) return EWPT_ERR_INVAL;
         if (ctxs[i]->device != ctxs[0]->device || ctxs[i]->precision != ctxs[0]->precision) return EWPT_ERR_INVAL;
+        if (ctxs[i]->zg) return EWPT_ERR_UNSUPPORTED;   // batches hold periodic lattices
         for (int j = 0; j < i; j++)
             if (ctxs[j] == ctxs[i]) return EWPT_ERR_INVAL;
     }

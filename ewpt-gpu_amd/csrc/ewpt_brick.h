@@ -54,6 +54,7 @@ struct BrickGeo {
     int xcd_map;            // 1: remap block ids so each XCD walks a compact region
     int order;              // brick enumeration, see brick_coords
     double inv60dx, invdx2;
+    int zg = 0;             // ghost planes below/above in z (StageBufs::zghost); 0 = periodic in z
 };
 
 constexpr int NPASS = 32;
@@ -295,7 +296,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
     const int tx = t % TX, ty = (t / TX) % TY, tz = t / (TX * TY);
     const long nxl = g.nx, nxy = (long)g.nx * g.ny;
-    const unsigned qb = (unsigned)((x0 + tx) + nxl * (y0 + ty) + nxy * (z0 + tz)) * (unsigned)sizeof(T);
+    const unsigned qb = (unsigned)((x0 + tx) + nxl * (y0 + ty) + nxy * (z0 + tz + g.zg)) * (unsigned)sizeof(T);
     auto Y = [&](int ch) { return b.y + (long)ch * b.n3; };
     auto K = [&](const T *k, int m) { return k + (long)m * b.n3; };
     const R half_dt = dt * R(0.5), d6 = dt / R(6), d3 = dt / R(3);
@@ -418,8 +419,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             if (f < face_count<TX, TY, TZ>(dir)) {
                 int lx, ly, lz;
                 face_elem<TX, TY, TZ>(dir, f, lx, ly, lz);
-                goff[dir][s] = (int)((wrapc(x0 + lx - 3, g.nx) + nxl * wrapc(y0 + ly - 3, g.ny) +
-                                      nxy * wrapc(z0 + lz - 3, g.nz)) * (long)sizeof(T));
+                // with ghost planes (g.zg >= 3) the z halo is read from them
+                const int zz = g.zg ? z0 + lz - 3 + g.zg : wrapc(z0 + lz - 3, g.nz);
+                goff[dir][s] = (int)((wrapc(x0 + lx - 3, g.nx) + nxl * wrapc(y0 + ly - 3, g.ny) + nxy * zz) *
+                                     (long)sizeof(T));
                 loff[dir][s] = lz * B::HX * B::HY + ly * B::HX + lx;
             }
         }
@@ -786,6 +789,7 @@ hipError_t launch_tile(int stage, const StageBufs<T> &b, const ewpt_lattice &L, 
     g.xcd_map = (xcd_map && nb % 8 == 0) ? 1 : 0;
     g.inv60dx = 1.0 / (60.0 * L.dx);
     g.invdx2 = 1.0 / (L.dx * L.dx);
+    g.zg = b.zghost;
     if (P.bw != 0.0 || P.by != 0.0) return launch_tile_beta<TX, TY, TZ, PF, true, T>(stage, b, g, P, T(dt), st);
     return launch_tile_beta<TX, TY, TZ, PF, false, T>(stage, b, g, P, T(dt), st);
 }

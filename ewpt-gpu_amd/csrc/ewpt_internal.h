@@ -62,6 +62,18 @@ hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream
 hipError_t launch_join(const float *hi, const float *lo, double *y, long n, hipStream_t st);
 hipError_t launch_copy(const double *a, double *b, long n, hipStream_t st);
 
+// ewpt_slab.hip -- ghost planes of z-slab contexts: HALO planes below and
+// above the nz interior planes of every array (the stencil radius)
+constexpr int HALO = 3;
+hipError_t launch_halo_pack(const double *a, long n3, long plane, int nz, int nch, double *lo, double *hi,
+                            hipStream_t st);
+hipError_t launch_halo_pack(const float *a, long n3, long plane, int nz, int nch, float *lo, float *hi,
+                            hipStream_t st);
+hipError_t launch_halo_unpack(double *a, long n3, long plane, int nz, int nch, const double *lo, const double *hi,
+                              hipStream_t st);
+hipError_t launch_halo_unpack(float *a, long n3, long plane, int nz, int nch, const float *lo, const float *hi,
+                              hipStream_t st);
+
 // stage kernel variants selectable through ewpt_set_variant
 constexpr int VARIANT_AUTO = 0;
 constexpr int VARIANT_GATHER = 9;

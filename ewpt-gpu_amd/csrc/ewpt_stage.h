@@ -32,6 +32,10 @@ struct StageBufs {
     // stage's input (36 * n3; stage 1 reads y), unext receives the next one
     const T *u;
     T *unext;
+    // z-slab of a lattice split over several contexts: every array holds
+    // zghost planes below and above the nz interior planes (the neighbours'
+    // halo, filled by the exchange) and is not periodic in z; 0 = periodic
+    int zghost = 0;
 };
 
 // Value of channel c of the stage-S input at flat site q.

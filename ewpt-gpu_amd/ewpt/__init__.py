@@ -7,8 +7,9 @@ from ._lib import (EWPT_FP64, EWPT_MIXED, EwptError, EwptLattice, EwptParams, NC
 from .batch import Batch
 from .corrector import Corrector, make_pair
 from .ic import bubble_table, gen_phi
-from .lattice import Lattice, as_params
+from .lattice import Lattice, as_params, slab_step, slab_step_timed
 from .parameters import Parameters, J_ext, damp
 
 __all__ = ["EWPT_FP64", "EWPT_MIXED", "EwptError", "EwptLattice", "EwptParams", "NCH", "NK", "load",
-           "Batch", "Corrector", "make_pair", "Lattice", "as_params", "bubble_table", "gen_phi", "Parameters", "J_ext", "damp"]
+           "Batch", "Corrector", "make_pair", "Lattice", "as_params", "slab_step", "slab_step_timed", "bubble_table",
+           "gen_phi", "Parameters", "J_ext", "damp"]

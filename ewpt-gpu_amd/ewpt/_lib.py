@@ -38,7 +38,9 @@ EXPORTS = (
     "ewpt_copy_state", "ewpt_net_create", "ewpt_net_destroy", "ewpt_net_last_error", "ewpt_net_param_count",
     "ewpt_net_set_params", "ewpt_net_get_params", "ewpt_net_get_grads", "ewpt_net_set_norm", "ewpt_net_apply",
     "ewpt_net_train_step", "ewpt_step_corrected", "ewpt_net_conv_test", "ewpt_net_wgrad_test",
-    "ewpt_net_set_input", "ewpt_copy_probe",
+    "ewpt_net_set_input", "ewpt_copy_probe", "ewpt_create_slab", "ewpt_slab_halo_bytes", "ewpt_slab_buffers",
+    "ewpt_slab_pack", "ewpt_slab_unpack", "ewpt_stage", "ewpt_get_stream", "ewpt_sync", "ewpt_slab_step",
+    "ewpt_slab_step_timed",
 )
 
 
@@ -120,6 +122,16 @@ def load() -> ctypes.CDLL:
         "ewpt_net_wgrad_test": [ip, ip, ip, hp, hp, fp, fp],
         "ewpt_spectrum": [vp, pp, ctypes.c_double, ip, ctypes.POINTER(ctypes.c_int), ip, dp, dp,
                           ctypes.POINTER(ctypes.c_int)],
+        "ewpt_create_slab": [ctypes.POINTER(vp), lp, ip, ip],
+        "ewpt_slab_halo_bytes": [vp, ip, ctypes.POINTER(ctypes.c_ulonglong)],
+        "ewpt_slab_buffers": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)],
+        "ewpt_slab_pack": [vp, ip, vp, vp],
+        "ewpt_slab_unpack": [vp, ip, vp, vp],
+        "ewpt_stage": [vp, pp, ip],
+        "ewpt_get_stream": [vp, ctypes.POINTER(vp)],
+        "ewpt_sync": [vp],
+        "ewpt_slab_step": [ctypes.POINTER(vp), ip, pp, ip],
+        "ewpt_slab_step_timed": [ctypes.POINTER(vp), ip, pp, ip, dp, dp, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -34,18 +34,24 @@ def as_params(p) -> EwptParams:
 
 
 class Lattice:
-    """A periodic nx*ny*nz lattice of the electroweak fields on one GPU."""
+    """A periodic nx*ny*nz lattice of the electroweak fields on one GPU.
 
-    def __init__(self, n, dx: float, precision: int = _lib.EWPT_FP64, device: int = 0):
+    ``slab=True``: one z-slab of a lattice split over several contexts (its
+    share nx*ny*nz of the z axis, with ghost planes exchanged before each
+    stage; see ``slab_step`` and the slab_* methods)."""
+
+    def __init__(self, n, dx: float, precision: int = _lib.EWPT_FP64, device: int = 0, slab: bool = False):
         self.nx, self.ny, self.nz = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
         self.dx = float(dx)
         self.precision = precision
+        self.slab = bool(slab)
         self._L = _lib.load()
         self._lat = EwptLattice(self.nx, self.ny, self.nz, self.dx)
         h = ctypes.c_void_p()
-        rc = self._L.ewpt_create(ctypes.byref(h), ctypes.byref(self._lat), precision, device)
+        create = self._L.ewpt_create_slab if slab else self._L.ewpt_create
+        rc = create(ctypes.byref(h), ctypes.byref(self._lat), precision, device)
         if rc != 0:
-            raise EwptError(rc, "ewpt_create")
+            raise EwptError(rc, "ewpt_create_slab" if slab else "ewpt_create")
         self._h = h
 
     # ---- lifetime
@@ -215,6 +221,46 @@ class Lattice:
     def variant(self, v: int):
         self._check(self._L.ewpt_set_variant(self._h, int(v)), "ewpt_set_variant")
 
+    # ---- z-slabs and asynchronous stage control
+    def halo_bytes(self, what: int = 0) -> int:
+        """Bytes of one side's halo buffer: what = 0 state, 1..3 k_s."""
+        b = ctypes.c_ulonglong()
+        self._check(self._L.ewpt_slab_halo_bytes(self._h, int(what), ctypes.byref(b)), "ewpt_slab_halo_bytes")
+        return b.value
+
+    def slab_buffers(self) -> tuple[int, int]:
+        """Device addresses of the context's own send buffers (low, high)."""
+        lo, hi = ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(self._L.ewpt_slab_buffers(self._h, ctypes.byref(lo), ctypes.byref(hi)), "ewpt_slab_buffers")
+        return lo.value, hi.value
+
+    def slab_pack(self, what: int, send_lo: int, send_hi: int):
+        """Pack the 3 lowest / highest interior planes of array `what` into
+        the device buffers (asynchronous on the context stream)."""
+        self._check(self._L.ewpt_slab_pack(self._h, int(what), ctypes.c_void_p(send_lo), ctypes.c_void_p(send_hi)),
+                    "ewpt_slab_pack")
+
+    def slab_unpack(self, what: int, recv_lo: int, recv_hi: int):
+        """Fill the ghost planes of array `what` from the device buffers
+        received from the lower / upper neighbour (asynchronous)."""
+        self._check(self._L.ewpt_slab_unpack(self._h, int(what), ctypes.c_void_p(recv_lo), ctypes.c_void_p(recv_hi)),
+                    "ewpt_slab_unpack")
+
+    def stage(self, p, s: int):
+        """Launch stage s (1..4) of a step, without exchange (asynchronous)."""
+        P = as_params(p)
+        self._check(self._L.ewpt_stage(self._h, ctypes.byref(P), int(s)), "ewpt_stage")
+
+    @property
+    def stream(self) -> int:
+        """The context's HIP stream handle."""
+        st = ctypes.c_void_p()
+        self._check(self._L.ewpt_get_stream(self._h, ctypes.byref(st)), "ewpt_get_stream")
+        return st.value or 0
+
+    def sync(self):
+        self._check(self._L.ewpt_sync(self._h), "ewpt_sync")
+
     @property
     def active_variant(self) -> int:
         """The stage variant the next step launches (what 0 resolves to)."""
@@ -226,3 +272,30 @@ class Lattice:
         b = ctypes.c_ulonglong()
         self._check(self._L.ewpt_device_bytes(self._h, ctypes.byref(b)), "ewpt_device_bytes")
         return b.value
+
+
+def _ring(slabs):
+    if not slabs:
+        raise ValueError("no slabs")
+    arr = (ctypes.c_void_p * len(slabs))(*[x._h for x in slabs])
+    return arr, slabs[0]
+
+
+def slab_step(slabs, p, nsteps: int = 1):
+    """nsteps RK4 steps of slab lattices stacked along z (slabs[0] lowest,
+    periodic over the ring), exchanging halos with device copies."""
+    arr, s0 = _ring(slabs)
+    P = as_params(p)
+    s0._check(s0._L.ewpt_slab_step(arr, len(slabs), ctypes.byref(P), int(nsteps)), "ewpt_slab_step")
+
+
+def slab_step_timed(slabs, p, nsteps: int):
+    """Like slab_step; returns (mean kernel ms per stage [4], mean exchange
+    ms per step, total ms), measured on slabs[0]'s stream."""
+    arr, s0 = _ring(slabs)
+    P = as_params(p)
+    st = np.zeros(4)
+    ex, tot = ctypes.c_double(), ctypes.c_double()
+    s0._check(s0._L.ewpt_slab_step_timed(arr, len(slabs), ctypes.byref(P), int(nsteps), _dptr(st),
+                                         ctypes.byref(ex), ctypes.byref(tot)), "ewpt_slab_step_timed")
+    return st.tolist(), ex.value, tot.value

@@ -139,9 +139,57 @@ int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
 
 /* Like ewpt_step, and reports mean device time per launch (ms) of each
  * stage kernel, measured with HIP events on the context stream:
- * stage_ms[0..3] = stages 1..4.  *total_ms = event time of the whole run. */
+ * stage_ms[0..3] = stages 1..4.  *total_ms = event time of the whole run.
+ * On a slab context each stage's time includes the halo exchange before it. */
 int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps,
                     double stage_ms[4], double *total_ms);
+
+/* ---- z-slabs: one lattice split over several contexts (one per GPU).
+ * No reference counterpart on the single-GPU path; the reference's own
+ * multi-GPU driver (main_rk4_multiple.jl:69-213, which its README says no
+ * longer works) exchanges boundary planes with MPI.  A slab context holds the
+ * local lattice nx x ny x nz (nz = its share of the z axis) with 3 ghost
+ * planes below and above in every array; its stage kernels read the z halo
+ * from the ghost planes instead of wrapping.  Before stage s the planes that
+ * stage reads at halo sites are exchanged: the state before stage 1 (what =
+ * 0; mixed: its FP32 high word) and k_{s-1} before stage s > 1 (what =
+ * s-1).  A multi-GPU driver runs, per stage and rank:
+ *   ewpt_slab_pack(ctx, what, send_lo, send_hi)   lowest / highest 3 interior planes
+ *   send send_lo to the lower and send_hi to the upper z-neighbour; receive
+ *   recv_lo from the lower neighbour (its send_hi), recv_hi from the upper
+ *   ewpt_slab_unpack(ctx, what, recv_lo, recv_hi)
+ *   ewpt_stage(ctx, p, s)
+ * pack, unpack and stage are asynchronous on the context's stream
+ * (ewpt_get_stream); buffers are device memory of ewpt_slab_halo_bytes each.
+ * ewpt_step on a slab context alone is periodic in z (it exchanges with
+ * itself).  Slab contexts take set/get/copy_state, steps, the brick stage
+ * variants without stored stage inputs and the calls below; the
+ * whole-lattice calls (ICs, RHS, energies, Gauss, EM fields, spectrum,
+ * corrector, batches) return EWPT_ERR_UNSUPPORTED on them. */
+int ewpt_create_slab(ewpt_ctx **out, const ewpt_lattice *local, int precision, int device);
+/* Bytes of one side's halo buffer for array `what` (0 state, 1..3 k_s). */
+int ewpt_slab_halo_bytes(ewpt_ctx *ctx, int what, unsigned long long *bytes);
+/* The context's own send buffers (device, large enough for what = 0). */
+int ewpt_slab_buffers(ewpt_ctx *ctx, void **send_lo, void **send_hi);
+int ewpt_slab_pack(ewpt_ctx *ctx, int what, void *send_lo, void *send_hi);
+int ewpt_slab_unpack(ewpt_ctx *ctx, int what, const void *recv_lo, const void *recv_hi);
+/* One stage launch (1..4) without exchange, asynchronous; stage 4 makes
+ * the new state current.  Any context. */
+int ewpt_stage(ewpt_ctx *ctx, const ewpt_params *p, int stage);
+/* The context's HIP stream (hipStream_t) and a wait for it. */
+int ewpt_get_stream(ewpt_ctx *ctx, void **stream);
+int ewpt_sync(ewpt_ctx *ctx);
+/* nsteps steps of n slab contexts on one device forming a ring along z
+ * (slab i's neighbours are i-1 and i+1 mod n), the exchange done with
+ * device copies between the contexts: the single-GPU stand-in for the
+ * exchange a multi-GPU driver does over xGMI.  All slabs share device,
+ * precision, nx, ny and dx; the result equals stepping the stacked lattice
+ * (slab 0 lowest) in one context with the same variant. */
+int ewpt_slab_step(ewpt_ctx *const *slabs, int n, const ewpt_params *p, int nsteps);
+/* Like ewpt_slab_step; on slab 0's stream: mean kernel time per stage,
+ * mean exchange time per step (the four exchanges) and the total. */
+int ewpt_slab_step_timed(ewpt_ctx *const *slabs, int n, const ewpt_params *p, int nsteps,
+                         double stage_ms[4], double *exchange_ms, double *total_ms);
 
 /* Energy table in the reference's column order (main_rk4.jl:6104-6116):
  * out[0..6] = PE, KE, GE, EEW, MEW, EEY, MEY (sum * dx^3, FP64),
