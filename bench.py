@@ -61,11 +61,14 @@ def stage_bytes_per_site(precision: str, variant: int) -> dict:
 
 STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 
-# Bytes per site that one stage launch moves across the L2/fabric boundary
-# at 512^3, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (profiles/stage_traffic_512.md): FETCH_SIZE x 2 (what that counter reads
-# for a plain device copy of known size) + WRITE_SIZE x 1.
-MEASURED_TRAFFIC = {}
+# Bytes per site that one stage launch of the automatic variants moves across
+# the L2/fabric boundary at 512^3, measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE on this bench: FETCH_SIZE x 2 (what that counter reads for a
+# plain device copy of known size) + WRITE_SIZE x 1.
+MEASURED_TRAFFIC = {
+    ("fp64", 311, 1): 849, ("fp64", 311, 2): 1630, ("fp64", 311, 3): 2212, ("fp64", 311, 4): 3725,
+    ("mixed", 315, 1): 652, ("mixed", 315, 2): 698, ("mixed", 315, 3): 698, ("mixed", 315, 4): 1364,
+}
 
 
 def measured_traffic(precision: str, variant: int, stage: int, n: int):
@@ -495,6 +498,7 @@ def main(argv=None) -> int:
             "variant": active,
             "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK, "traffic": measured_traffic(precision, active, dom, n),
+                         "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE)",
                          "kernel": f"stage {dom} ({precision}, variant {active})", "bytes_per_site": per_site[dom]},
             # the step against the minimal (formed-input) traffic of an RK4 step
             "step_roofline_frac": value * sum(STAGE_BYTES[precision].values()) / HBM_PEAK,
