@@ -1080,7 +1080,7 @@ void drop_plan(ewpt_batch *b)
 // candidate variants in preference order (measured single-lattice ranking)
 std::vector<int> batch_candidates(bool f32)
 {
-    std::vector<int> v = f32 ? std::vector<int>{311, 211, 411, 310, 210, 11, 10}
+    std::vector<int> v = f32 ? std::vector<int>{315, 215, 311, 211, 411, 310, 210, 15, 11, 10}
                              : std::vector<int>{311, 310, 211, 210, 303, 203, 11, 10, 3, 2};
     const size_t n = v.size();
     for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms
@@ -1108,7 +1108,12 @@ int ensure_plan(ewpt_batch *b)
     const int v = batch_resolved(b);
     if (v < 0) return bfail(b, EWPT_ERR_UNSUPPORTED, "no batch variant tiles every lattice");
     std::vector<ewpt_lattice> lats;
-    for (const ewpt_ctx *c : b->ctx) lats.push_back(c->lat);
+    for (ewpt_ctx *c : b->ctx) {
+        lats.push_back(c->lat);
+        // stored-stage-input variants need each context's stage-input buffer
+        if (int rc = ensure_stage_input_buffer(c, v % 1000))
+            return bfail(b, rc, ("batch: " + c->err).c_str());
+    }
     const char *what = "";
     b->plan = batch_plan_create(v, b->precision == EWPT_MIXED, lats, b->device, &what);
     if (!b->plan) return bfail(b, EWPT_ERR_HIP, what);
@@ -1116,7 +1121,7 @@ int ensure_plan(ewpt_batch *b)
 }
 
 template <typename T>
-void batch_lat(const ewpt_ctx *c, const ewpt_params &p, BatchLat<T> &L)
+void batch_lat(const ewpt_ctx *c, const ewpt_params &p, bool stored, BatchLat<T> &L)
 {
     std::memset(&L, 0, sizeof L);
     L.L = c->lat;
@@ -1133,11 +1138,13 @@ void batch_lat(const ewpt_ctx *c, const ewpt_params &p, BatchLat<T> &L)
                 sb.ylo_new = par ? c->flo : c->flo_n;
                 sb.k1 = c->fk[0]; sb.k2 = c->fk[1]; sb.k3 = c->fk[2];
                 sb.kout = s < 4 ? c->fk[s - 1] : nullptr;
+                if (stored) set_stage_inputs(sb, s, c->fu, par ? c->fhi : c->fhi_n);
             } else {
                 sb.y = par ? c->ynext : c->y;
                 sb.ynew = par ? c->y : c->ynext;
                 sb.k1 = c->k[0]; sb.k2 = c->k[1]; sb.k3 = c->k[2];
                 sb.kout = s < 4 ? c->k[s - 1] : nullptr;
+                if (stored) set_stage_inputs(sb, s, c->u, par ? c->y : c->ynext);
             }
         }
 }
@@ -1146,13 +1153,14 @@ void batch_lat(const ewpt_ctx *c, const ewpt_params &p, BatchLat<T> &L)
 int batch_upload(ewpt_batch *b, const ewpt_params *p)
 {
     int r;
+    const bool stored = brick_stored_inputs(batch_resolved(b) % 1000);
     if (b->precision == EWPT_MIXED) {
         std::vector<BatchLat<float>> v(b->n);
-        for (int i = 0; i < b->n; i++) batch_lat(b->ctx[i], p[i], v[i]);
+        for (int i = 0; i < b->n; i++) batch_lat(b->ctx[i], p[i], stored, v[i]);
         r = batch_plan_set(b->plan, v.data(), b->stream);
     } else {
         std::vector<BatchLat<double>> v(b->n);
-        for (int i = 0; i < b->n; i++) batch_lat(b->ctx[i], p[i], v[i]);
+        for (int i = 0; i < b->n; i++) batch_lat(b->ctx[i], p[i], stored, v[i]);
         r = batch_plan_set(b->plan, v.data(), b->stream);
     }
     return r ? bfail(b, EWPT_ERR_HIP, "batch descriptor upload") : EWPT_OK;
@@ -1609,6 +1617,7 @@ int ewpt_batch_tune(ewpt_batch *b, const ewpt_params *p, int nsteps, int *best_v
         if (rc == EWPT_OK) rc = ewpt_batch_step(b, q.data(), 2);   // warm-up: capture, first launches
         double ms = 0.0;
         if (rc == EWPT_OK) rc = ewpt_batch_step_timed(b, q.data(), nsteps, &ms);
+        if (rc == EWPT_ERR_OOM && brick_stored_inputs(v % 1000)) continue;   // no room for the stage inputs
         if (rc) return rc;
         if (ms < tbest) { tbest = ms; best = v; }
     }

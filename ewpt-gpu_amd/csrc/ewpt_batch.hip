@@ -79,7 +79,14 @@ bool tile_of(int v, bool f32, int &tx, int &ty, int &tz)
     case 11: tx = 32; ty = 4; tz = 4; return true;
     default: break;
     }
-    if (f32) return false;
+    if (f32) {
+        // stored stage inputs for stages 2-4 (the mixed path's best single-lattice form)
+        switch (v) {
+        case 14: tx = 16; ty = 8; tz = 4; return true;
+        case 15: tx = 32; ty = 4; tz = 4; return true;
+        default: return false;
+        }
+    }
     switch (v) {
     case 2: tx = 8; ty = 8; tz = 8; return true;
     case 3: tx = 16; ty = 8; tz = 4; return true;
@@ -189,6 +196,10 @@ template <typename T, typename F>
 auto with_kern(int v, F &&f)
 {
     if (v == 11) return f(Kern<32, 4, 4, 4432, T>{});
+    if constexpr (std::is_same_v<T, float>) {
+        if (v == 14) return f(Kern<16, 8, 4, 704432, T>{});
+        if (v == 15) return f(Kern<32, 4, 4, 704432, T>{});
+    }
     if constexpr (std::is_same_v<T, double>) {
         if (v == 2) return f(Kern<8, 8, 8, 1, T>{});
         if (v == 3) return f(Kern<16, 8, 4, 2, T>{});

@@ -297,9 +297,10 @@ int ewpt_batch_create(ewpt_batch **out, ewpt_ctx *const *ctxs, int n);
 int ewpt_batch_destroy(ewpt_batch *b);
 const char *ewpt_batch_last_error(ewpt_batch *b);
 /* Brick variant for every lattice (as ewpt_set_variant: 10/11 (+100*order)
- * for both precisions, also 2/3 for FP64), plus 1000 for the persistent
- * launch form (a resident grid striding through the bricks instead of one
- * workgroup per brick); 0 = first that tiles all. */
+ * for both precisions, also 2/3 for FP64 and the stored-stage-input forms
+ * 14/15 for mixed, which allocate each context's stage-input buffer), plus
+ * 1000 for the persistent launch form (a resident grid striding through the
+ * bricks instead of one workgroup per brick); 0 = first that tiles all. */
 int ewpt_batch_set_variant(ewpt_batch *b, int variant);
 int ewpt_batch_get_variant(ewpt_batch *b, int *variant);
 /* nsteps RK4 steps of every lattice; p holds one ewpt_params per lattice. */

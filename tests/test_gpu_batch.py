@@ -35,6 +35,7 @@ def _params(k):
 
 
 @pytest.mark.parametrize("precision,variant", [(ewpt.EWPT_MIXED, 210), (ewpt.EWPT_MIXED, 1010),
+                                               (ewpt.EWPT_MIXED, 214), (ewpt.EWPT_MIXED, 1014),
                                                (ewpt.EWPT_FP64, 210), (ewpt.EWPT_FP64, 1003)])
 def test_batch_bitwise_equals_single(precision, variant):
     shapes = [(32, 32, 32), (48, 48, 48), (64, 32, 16), (32, 48, 64)]
@@ -77,7 +78,7 @@ def test_batch_tune_leaves_states_and_caches():
             bt.step(fields.params(dt=0.0), 1)
             before = [a.get_state() for a in lats]
             v, ms = bt.tune(_params(0), 2)
-            assert v % 100 in (10, 11) and ms > 0
+            assert v % 100 in (10, 11, 14, 15) and ms > 0
             assert bt.variant == v
             for a, s in zip(lats, before):
                 assert np.array_equal(a.get_state(), s)      # dt = 0 timing steps change nothing
