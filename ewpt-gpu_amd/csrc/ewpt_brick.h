@@ -192,15 +192,22 @@ __device__ __forceinline__ R phiTY_coef(const R p[4])
 // Load/store at a 32-bit byte offset from a (uniform) channel base pointer:
 // lets the compiler use the SGPR-base + VGPR-offset addressing form instead
 // of materialising one 64-bit address per channel.
+// The accesses are made through global-address-space pointers: the batched
+// kernels take their buffer pointers from a descriptor in memory, which the
+// compiler would otherwise address as generic (FLAT) memory, and FLAT loads
+// also count against the LDS wait of lds_barrier, so every barrier would wait
+// for the prefetched halo loads.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T *;
 template <typename T>
 __device__ __forceinline__ T ldo(const T *base, unsigned boff)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+    return *(global_ptr<const T>)(reinterpret_cast<const char *>(base) + boff);
 }
 template <typename T>
 __device__ __forceinline__ void sto(T *base, unsigned boff, T v)
 {
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+    *(global_ptr<T>)(reinterpret_cast<char *>(base) + boff) = v;
 }
 // Streaming (non-temporal) store: no launch re-reads what it writes, so
 // its lines are evicted first and leave L2 to the halo lines that
@@ -209,7 +216,7 @@ __device__ __forceinline__ void sto(T *base, unsigned boff, T v)
 template <typename T>
 __device__ __forceinline__ void stn(T *base, unsigned boff, T v)
 {
-    __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff));
+    __builtin_nontemporal_store(v, (global_ptr<T>)(reinterpret_cast<char *>(base) + boff));
 }
 
 // Workgroup barrier for LDS only.  __syncthreads() also fences global
