@@ -17,6 +17,7 @@ Prints ONE JSON line (rank 0):
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -506,8 +507,15 @@ def main(argv=None) -> int:
             "energy_check": float(sum(e[:7])),
         }
 
+    # same-run copy ceiling: libewpt's one-double-per-lane copy of 4 GiB
+    ms_copy = ctypes.c_double(0.0)
+    rc = ewpt.load().ewpt_copy_probe(None, None, (4 << 30) // 8, 10, None, ctypes.byref(ms_copy))
+    copy_gbs = 2 * (4 << 30) / (ms_copy.value * 1e-3) / 1e9 if rc == 0 and ms_copy.value > 0 else None
+
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     res = {pr: run(pr, a.variant) for pr in precs}
+    for r in res.values():
+        r["roofline"]["copy_ceiling_GB_per_s"] = copy_gbs
     head = res[precs[0]]
     out = {
         "metric": METRIC,

@@ -491,7 +491,27 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host)
 
 int ewpt_copy_probe(const double *dev_src, double *dev_dst, long n, int reps, void *stream, double *ms)
 {
-    if (!dev_src || !dev_dst || n < 1 || reps < 1 || !ms) return EWPT_ERR_INVAL;
+    if ((!dev_src) != (!dev_dst) || n < 1 || reps < 1 || !ms) return EWPT_ERR_INVAL;
+    if (!dev_src) {
+        // own buffers
+        double *a = nullptr, *b = nullptr;
+        if (hipMalloc(reinterpret_cast<void **>(&a), n * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return EWPT_ERR_OOM;
+        }
+        if (hipMalloc(reinterpret_cast<void **>(&b), n * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(a);
+            return EWPT_ERR_OOM;
+        }
+        int rc = hipMemsetAsync(a, 0, n * sizeof(double), static_cast<hipStream_t>(stream)) == hipSuccess
+                     ? ewpt_copy_probe(a, b, n, reps, stream, ms)
+                     : EWPT_ERR_HIP;
+        (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+        (void)hipFree(a);
+        (void)hipFree(b);
+        return rc;
+    }
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return EWPT_ERR_HIP;

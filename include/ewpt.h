@@ -70,7 +70,9 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
 /* Measurement aid, no counterpart in the reference: a plain FP64 device copy
  * of n doubles with one 8-byte access per lane (the stage kernels' access
  * width), launched reps + 1 times on stream; *ms = mean time of the last
- * reps.  The reference point for memory-traffic counters. */
+ * reps.  The reference point for memory-traffic counters.  With dev_src and
+ * dev_dst both NULL the library allocates (and frees) the two buffers on
+ * the current device. */
 int ewpt_copy_probe(const double *dev_src, double *dev_dst, long n, int reps, void *stream, double *ms);
 
 /* Device-to-device copy of the state of src into dst (same device,
