@@ -32,7 +32,7 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6}
+STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:
@@ -67,8 +67,9 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # and WRITE_SIZE on this bench: FETCH_SIZE x 2 (what that counter reads for a
 # plain device copy of known size) + WRITE_SIZE x 1.
 MEASURED_TRAFFIC = {
+    ("fp64", 321, 1): 1113, ("fp64", 321, 2): 1395, ("fp64", 321, 3): 1415, ("fp64", 321, 4): 2349,
     ("fp64", 311, 1): 849, ("fp64", 311, 2): 1630, ("fp64", 311, 3): 2212, ("fp64", 311, 4): 3725,
-    ("mixed", 315, 1): 652, ("mixed", 315, 2): 698, ("mixed", 315, 3): 698, ("mixed", 315, 4): 1364,
+    ("mixed", 315, 1): 631, ("mixed", 315, 2): 698, ("mixed", 315, 3): 701, ("mixed", 315, 4): 1360,
 }
 
 

@@ -248,6 +248,7 @@ __host__ __device__ constexpr int eps3(int a, int b) { return ((b - a + 3) % 3 =
 template <typename T, int MAXF>
 struct PassRegs {
     T v[MAXF][3];
+    T vc[3];    // the site's own value, for passes that load their centre
 };
 
 // Prefetch depth of stage S from the profile PF: a single digit applies to
@@ -268,7 +269,15 @@ __host__ __device__ constexpr bool paired_of(int PF, int S) { return PF % 100000
 // reads its input u_s as one array (every staged value, centre or halo, is a
 // single load instead of two or three), and stage s-1 writes it (36 channels:
 // y + c k_{s-1}, the fields' k being the stage momenta).
-__host__ __device__ constexpr bool stored_in(int PF, int S) { return S >= 2 && ((PF / 100000) >> (S - 2)) & 1; }
+__host__ __device__ constexpr bool stored_in(int PF, int S) { return S >= 2 && (((PF / 100000) % 10) >> (S - 2)) & 1; }
+// PF / 1000000 = 1: late centres.  The E and Gamma/Sigma centres (state
+// channels 16..19, 24..35) are not loaded up front but with their own pass,
+// so a brick reads its interior of such a channel while its neighbours read
+// the same lines as their halo; the terms that need these centres earlier
+// (W x E in k_Gamma, W x Gamma in kt_W, the Gamma/Sigma generator in kt_phi)
+// are added once their passes are done.
+__host__ __device__ constexpr bool late_centres(int PF) { return (PF / 1000000) % 10 == 1; }
+__host__ __device__ constexpr bool is_late_channel(int ch) { return (ch >= 16 && ch < 20) || ch >= 24; }
 __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
@@ -359,6 +368,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     R c[NCH];
 #pragma unroll
     for (int ch = 0; ch < NCH; ch++) {
+        if (late_centres(PF) && is_late_channel(ch)) {
+            c[ch] = R(0);   // set when the channel's pass is staged
+            continue;
+        }
         const R yv = ldo(US ? U(ch) : Y(ch), qb);
         if constexpr (S == 1 || US) {
             c[ch] = yv;
@@ -386,22 +399,30 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     R Bw[3][3], BYv[3] = {R(0), R(0), R(0)};
     {
         const R pot = R(2) * R(P.lam) * (r2 - R(P.vev) * R(P.vev)) + R(0.5) * R(P.jex);
-        R Gp[4];
-        apply_gen(h, c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2], hy, c[CH_SIG], p, Gp);
+        R Gp[4] = {R(0), R(0), R(0), R(0)};
+        if constexpr (!late_centres(PF)) apply_gen(h, c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2], hy, c[CH_SIG], p, Gp);
         const R inv_r2 = R(1) / r2;   // keeps the reference's 0/0 at |phi| = 0
 #pragma unroll
-        for (int bb = 0; bb < 4; bb++)
-            ktphi[bb] = Gp[bb] - pot * p[bb] - R(P.gam) * p[bb] * (p[bb] * c[CH_PI + bb]) * inv_r2;
+        for (int bb = 0; bb < 4; bb++) {
+            ktphi[bb] = -pot * p[bb] - R(P.gam) * p[bb] * (p[bb] * c[CH_PI + bb]) * inv_r2;
+            if constexpr (!late_centres(PF)) ktphi[bb] = Gp[bb] + ktphi[bb];
+        }
         const R pi[4] = {c[CH_PI], c[CH_PI + 1], c[CH_PI + 2], c[CH_PI + 3]};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            R na = R(0);
+            if constexpr (late_centres(PF)) {
+                kG[a] = gp2 * gw * phiT(a, p, pi);
 #pragma unroll
-            for (int i = 0; i < 3; i++) na -= xprod(a, W(0, i), W(1, i), W(2, i), E(0, i), E(1, i), E(2, i));
-            kG[a] = gp2 * gw * (na + phiT(a, p, pi));
+                for (int i = 0; i < 3; i++) ktW[a][i] = R(0);
+            } else {
+                R na = R(0);
 #pragma unroll
-            for (int i = 0; i < 3; i++)
-                ktW[a][i] = -gw * xprod(a, W(0, i), W(1, i), W(2, i), c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2]);
+                for (int i = 0; i < 3; i++) na -= xprod(a, W(0, i), W(1, i), W(2, i), E(0, i), E(1, i), E(2, i));
+                kG[a] = gp2 * gw * (na + phiT(a, p, pi));
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    ktW[a][i] = -gw * xprod(a, W(0, i), W(1, i), W(2, i), c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2]);
+            }
         }
         kS = gp2 * gy * phiTY(p, pi);
 #pragma unroll
@@ -437,6 +458,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     const int lcen = (tz + 3) * B::HX * B::HY + (ty + 3) * B::HX + (tx + 3);
 
     using Regs = PassRegs<T, MAXF>;
+    // passes whose centre value is loaded with the pass (late_centres)
+    constexpr auto pass_loads_centre = [](int pp) constexpr {
+        return late_centres(PF) && is_late_channel(pass_channel(pp));
+    };
     auto fetch = [&](auto pc, Regs &rg) {
         constexpr int pp = decltype(pc)::value;
         constexpr int ch = pass_channel(pp);
@@ -463,6 +488,11 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 if constexpr (S > 2 && !US && ch < 16) rg.v[s][2] = ldo(k2p, o);
             }
         }
+        if constexpr (pass_loads_centre(pp)) {
+            rg.vc[0] = ldo(y0p, qb);
+            if constexpr (S > 1 && !US) rg.vc[1] = ldo(y1p, qb);
+            if constexpr (S > 2 && !US && ch < 16) rg.vc[2] = ldo(k2p, qb);
+        }
     };
     auto stage_value = [&](auto pc, const T (&v)[3]) -> R {
         constexpr int ch = pass_channel(decltype(pc)::value);
@@ -479,7 +509,13 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         constexpr int pp = decltype(pc)::value;
         constexpr int dir = pass_dir(pp);
         constexpr int nf = face_count<TX, TY, TZ>(dir);
-        buf[lcen] = c[pass_channel(pp)];
+        if constexpr (pass_loads_centre(pp)) {
+            const R cv = stage_value(pc, rg.vc);
+            c[pass_channel(pp)] = cv;
+            buf[lcen] = cv;
+        } else {
+            buf[lcen] = c[pass_channel(pp)];
+        }
 #pragma unroll
         for (int s = 0; s < MAXF; s++)
             if (s * B::NT < nf) buf[loff[dir][s]] = stage_value(pc, rg.v[s]);
@@ -508,6 +544,17 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             deriv(buf, i, cen, false, d1, d2);
             if constexpr (a < 3) kG[a] += (R(1) - gp2) * d1;
             else kS += (R(1) - gp2) * d1;
+            if constexpr (late_centres(PF) && pp == 11) {
+                // every E centre is staged: the W x E term of k_Gamma
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) {
+                    R na = R(0);
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++)
+                        na -= xprod(aa, W(0, ii), W(1, ii), W(2, ii), E(0, ii), E(1, ii), E(2, ii));
+                    kG[aa] += gp2 * gw * na;
+                }
+            }
         } else if constexpr (kind == 1) {
             // Gamma^a, Sigma: -d_i Gamma^a into kt_W^a_i, -d_i Sigma into kt_Y_i
 #pragma unroll
@@ -516,6 +563,19 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 deriv(buf, i, cen, false, d1, d2);
                 if constexpr (pp < 15) ktW[pp - 12][i] -= d1;
                 else ktY[i] -= d1;
+            }
+            if constexpr (late_centres(PF) && pp == 15) {
+                // every Gamma/Sigma centre is staged: W x Gamma in kt_W, the generator in kt_phi
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++)
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++)
+                        ktW[aa][ii] -= gw * xprod(aa, W(0, ii), W(1, ii), W(2, ii), c[CH_GAM], c[CH_GAM + 1],
+                                                  c[CH_GAM + 2]);
+                R Gp[4];
+                apply_gen(h, c[CH_GAM], c[CH_GAM + 1], c[CH_GAM + 2], hy, c[CH_SIG], p, Gp);
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++) ktphi[bb] += Gp[bb];
             }
         } else if constexpr (kind == 2) {
             // phi_b: kt_phi (Laplacian, A_i(d_i phi + D_i phi)), phi.T D into kt_W, kt_Y, u_j

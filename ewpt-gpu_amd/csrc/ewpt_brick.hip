@@ -16,6 +16,9 @@
 //   14, 15: as 10, 11 with stored stage inputs for stages 2-4 (brick_stored_inputs)
 //   16, 17: as 10, 11 with a stored stage input for stage 4
 //   18, 19: as 10, 11 with stored stage inputs for stages 3-4
+//   20, 22: as 11, 10 with the E and Gamma/Sigma centres loaded with their
+//       passes (late_centres); 21, 23: the same with stored stage inputs for
+//       stages 2-4
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -85,18 +88,34 @@ bool brick_fits_v19(const ewpt_lattice &L);
 hipError_t launch_brick_v19(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
-int brick_variant_count() { return 19; }
+bool brick_fits_v20(const ewpt_lattice &L);
+hipError_t launch_brick_v20(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v21(const ewpt_lattice &L);
+hipError_t launch_brick_v21(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v22(const ewpt_lattice &L);
+hipError_t launch_brick_v22(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v23(const ewpt_lattice &L);
+hipError_t launch_brick_v23(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+int brick_variant_count() { return 23; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 19 && v != 9 && order <= 4;
+    return v >= 1 && v <= 23 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)
 {
     const int v = variant % 100;
-    return variant >= 0 && v >= 14 && v <= 19;
+    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23);
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)
@@ -120,6 +139,10 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 17: return brick_fits_v17(L);
     case 18: return brick_fits_v18(L);
     case 19: return brick_fits_v19(L);
+    case 20: return brick_fits_v20(L);
+    case 21: return brick_fits_v21(L);
+    case 22: return brick_fits_v22(L);
+    case 23: return brick_fits_v23(L);
     default: return false;
     }
 }
@@ -148,6 +171,10 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 17: return launch_brick_v17(stage, b, L, P, dt, order, st);
     case 18: return launch_brick_v18(stage, b, L, P, dt, order, st);
     case 19: return launch_brick_v19(stage, b, L, P, dt, order, st);
+    case 20: return launch_brick_v20(stage, b, L, P, dt, order, st);
+    case 21: return launch_brick_v21(stage, b, L, P, dt, order, st);
+    case 22: return launch_brick_v22(stage, b, L, P, dt, order, st);
+    case 23: return launch_brick_v23(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -6,6 +6,8 @@
 //   14, 15 (+100*order): as 10, 11 with stored stage inputs for stages 2-4
 //   16, 17 (+100*order): as 10, 11 with a stored stage input for stage 4
 //   18, 19 (+100*order): as 10, 11 with stored stage inputs for stages 3-4
+//   20, 22 (+100*order): as 11, 10 with late E and Gamma/Sigma centres
+//   21, 23 (+100*order): 20, 22 with stored stage inputs for stages 2-4
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -23,6 +25,10 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 17: return brick_fits<32, 4, 4>(L);
     case 18: return brick_fits<16, 8, 4>(L);
     case 19: return brick_fits<32, 4, 4>(L);
+    case 20: return brick_fits<32, 4, 4>(L);
+    case 21: return brick_fits<32, 4, 4>(L);
+    case 22: return brick_fits<16, 8, 4>(L);
+    case 23: return brick_fits<16, 8, 4>(L);
     default: return false;
     }
 }
@@ -42,6 +48,10 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 17: return launch_tile<32, 4, 4, 404432, float>(stage, b, L, P, dt, 1, order, st);
     case 18: return launch_tile<16, 8, 4, 604432, float>(stage, b, L, P, dt, 1, order, st);
     case 19: return launch_tile<32, 4, 4, 604432, float>(stage, b, L, P, dt, 1, order, st);
+    case 20: return launch_tile<32, 4, 4, 1004432, float>(stage, b, L, P, dt, 1, order, st);
+    case 21: return launch_tile<32, 4, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 22: return launch_tile<16, 8, 4, 1004432, float>(stage, b, L, P, dt, 1, order, st);
+    case 23: return launch_tile<16, 8, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -250,9 +250,10 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 /* Kernel variant of the stage launches.  All variants compute the same
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
- *   0 = automatic: the first of 311, 310, 210, 211, 2 that tiles the
- *       lattice, else 9 (mixed: 315, 215, 311, 211, 310, 210; 315 and 215
- *       only when the stage-input buffer fits in device memory)
+ *   0 = automatic: the first of 321, 320, 311, 310, 210, 211, 2 that tiles
+ *       the lattice, else 9 (mixed: 315, 215, 311, 211, 310, 210); the
+ *       stored-stage-input variants (321, 315, 215) only when their buffer
+ *       fits in device memory and not on slab contexts
  *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
  *          5 = 16x8x4 pf1 linear order, 6 = 16x4x4 pf1, 7 = 8x8x4 pf2,
@@ -261,9 +262,12 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *          per LDS barrier; 14, 15 = 10, 11 with stored stage inputs
  *          (stages 1-3 also write the next stage's input, 36 channels,
  *          which stages 2-4 read as one array); 16, 17 = the same for
- *          stage 4 only; 18, 19 = for stages 3 and 4.  14..19 hold one more
- *          state-sized buffer (allocated when first selected).  All but 5
- *          use the XCD-aware block order.  The mixed path takes 10..19.
+ *          stage 4 only; 18, 19 = for stages 3 and 4; 20, 22 = 11, 10 with
+ *          the E and Gamma/Sigma centre values loaded with their own
+ *          passes; 21, 23 = 20, 22 with stored stage inputs for stages 2-4.
+ *          14..19, 21, 23 hold one more state-sized buffer (allocated when
+ *          first selected).  All but 5 use the XCD-aware block order.  The
+ *          mixed path takes 10..23.
  *   9 = gather kernel (one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
