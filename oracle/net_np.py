@@ -2,8 +2,8 @@
 
 Only ``tests/`` use this module, as the checker of the device convolution,
 weight-gradient and training kernels (ewpt_net.hip).  No reference
-counterpart: the network is this build's own (36 -> 64 -> 64 -> 36 periodic
-3x3x3 convolutions with SiLU).
+counterpart: the reference has no learned component; the network here is
+36 -> 64 -> 64 -> 36 periodic 3x3x3 convolutions with SiLU.
 
 Layout as on the device: activations (nz, ny, nx, 64) channel-last, weights
 [27 taps][64 out][64 in] with tap = (dz+1)*9 + (dy+1)*3 + (dx+1) and
