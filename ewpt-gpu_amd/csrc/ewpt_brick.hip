@@ -19,6 +19,8 @@
 //   20, 22: as 11, 10 with the E and Gamma/Sigma centres loaded with their
 //       passes (late_centres); 21, 23: the same with stored stage inputs for
 //       stages 2-4
+//   24, 25: 16x4x4 bricks of 256 threads (two workgroups per CU in FP64) as
+//       21 and 20; 26: 8x8x4 bricks of 256 threads as 21
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -104,18 +106,30 @@ bool brick_fits_v23(const ewpt_lattice &L);
 hipError_t launch_brick_v23(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
-int brick_variant_count() { return 23; }
+bool brick_fits_v24(const ewpt_lattice &L);
+hipError_t launch_brick_v24(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v25(const ewpt_lattice &L);
+hipError_t launch_brick_v25(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+bool brick_fits_v26(const ewpt_lattice &L);
+hipError_t launch_brick_v26(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+int brick_variant_count() { return 26; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 23 && v != 9 && order <= 4;
+    return v >= 1 && v <= 26 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)
 {
     const int v = variant % 100;
-    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23);
+    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23 || v == 24 || v == 26);
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)
@@ -143,6 +157,9 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 21: return brick_fits_v21(L);
     case 22: return brick_fits_v22(L);
     case 23: return brick_fits_v23(L);
+    case 24: return brick_fits_v24(L);
+    case 25: return brick_fits_v25(L);
+    case 26: return brick_fits_v26(L);
     default: return false;
     }
 }
@@ -175,6 +192,9 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 21: return launch_brick_v21(stage, b, L, P, dt, order, st);
     case 22: return launch_brick_v22(stage, b, L, P, dt, order, st);
     case 23: return launch_brick_v23(stage, b, L, P, dt, order, st);
+    case 24: return launch_brick_v24(stage, b, L, P, dt, order, st);
+    case 25: return launch_brick_v25(stage, b, L, P, dt, order, st);
+    case 26: return launch_brick_v26(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }

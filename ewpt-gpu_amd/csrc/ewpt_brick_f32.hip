@@ -8,6 +8,8 @@
 //   18, 19 (+100*order): as 10, 11 with stored stage inputs for stages 3-4
 //   20, 22 (+100*order): as 11, 10 with late E and Gamma/Sigma centres
 //   21, 23 (+100*order): 20, 22 with stored stage inputs for stages 2-4
+//   24, 25 (+100*order): 16x4x4 bricks of 256 threads as 21, 20
+//   26 (+100*order): 8x8x4 bricks of 256 threads as 21
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -29,6 +31,9 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 21: return brick_fits<32, 4, 4>(L);
     case 22: return brick_fits<16, 8, 4>(L);
     case 23: return brick_fits<16, 8, 4>(L);
+    case 24: return brick_fits<16, 4, 4>(L);
+    case 25: return brick_fits<16, 4, 4>(L);
+    case 26: return brick_fits<8, 8, 4>(L);
     default: return false;
     }
 }
@@ -52,6 +57,9 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 21: return launch_tile<32, 4, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
     case 22: return launch_tile<16, 8, 4, 1004432, float>(stage, b, L, P, dt, 1, order, st);
     case 23: return launch_tile<16, 8, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 24: return launch_tile<16, 4, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 25: return launch_tile<16, 4, 4, 1004432, float>(stage, b, L, P, dt, 1, order, st);
+    case 26: return launch_tile<8, 8, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }
