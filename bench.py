@@ -67,9 +67,10 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # and WRITE_SIZE on this bench: FETCH_SIZE x 2 (what that counter reads for a
 # plain device copy of known size) + WRITE_SIZE x 1.
 MEASURED_TRAFFIC = {
+    ("fp64", 324, 1): 1154, ("fp64", 324, 2): 1387, ("fp64", 324, 3): 1386, ("fp64", 324, 4): 2407,
     ("fp64", 321, 1): 1113, ("fp64", 321, 2): 1395, ("fp64", 321, 3): 1415, ("fp64", 321, 4): 2349,
     ("fp64", 311, 1): 849, ("fp64", 311, 2): 1630, ("fp64", 311, 3): 2212, ("fp64", 311, 4): 3725,
-    ("mixed", 315, 1): 631, ("mixed", 315, 2): 698, ("mixed", 315, 3): 701, ("mixed", 315, 4): 1360,
+    ("mixed", 315, 1): 628, ("mixed", 315, 2): 696, ("mixed", 315, 3): 698, ("mixed", 315, 4): 1358,
 }
 
 
