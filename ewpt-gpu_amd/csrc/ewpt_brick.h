@@ -278,6 +278,9 @@ __host__ __device__ constexpr bool stored_in(int PF, int S) { return S >= 2 && (
 // are added once their passes are done.
 __host__ __device__ constexpr bool late_centres(int PF) { return (PF / 1000000) % 10 == 1; }
 __host__ __device__ constexpr bool is_late_channel(int ch) { return (ch >= 16 && ch < 20) || ch >= 24; }
+// PF / 10000000 = 1: stages 2-3 that write the next stage input load the y
+// values it needs during their last passes instead of after them.
+__host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10 == 1; }
 __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
@@ -708,11 +711,20 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     [&]<int... I>(std::integer_sequence<int, I...>) {
         (fetch(std::integral_constant<int, I>{}, rr[I]), ...);
     }(std::make_integer_sequence<int, PFS>{});
+    // y for the next stage input, loaded during the last YP passes (y_early)
+    constexpr bool YE = y_early(PF) && S > 1 && stores_next(PF, S) && !paired_of(PF, S);
+    constexpr int YP = 8;
+    R ypre[YE ? NCH : 1];
     if constexpr (!paired_of(PF, S)) {
         store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
         lds_barrier();
         auto body = [&](auto pc) {
             constexpr int pp = decltype(pc)::value;
+            if constexpr (YE && pp >= NPASS - YP) {
+                constexpr int q = pp - (NPASS - YP);
+#pragma unroll
+                for (int ch = q * NCH / YP; ch < (q + 1) * NCH / YP; ch++) ypre[ch] = ldo(Y(ch), qb);
+            }
             if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
             consume(pc, lds[pp & 1]);
             pin_all(pc);
@@ -772,7 +784,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             const R cc = S == 3 ? dt : half_dt;
 #pragma unroll
             for (int ch = 0; ch < NCH; ch++) {
-                const R yv = S == 1 ? c[ch] : R(ldo(Y(ch), qb));
+                R yv;
+                if constexpr (S == 1) yv = c[ch];
+                else if constexpr (YE) yv = ypre[ch];
+                else yv = R(ldo(Y(ch), qb));
                 const R kv = ch < 16 ? c[ch + 20] : o[ch - 16];
                 stn(b.unext + (long)ch * b.n3, qb, yv + kv * cc);
             }

@@ -21,6 +21,8 @@
 //       stages 2-4
 //   24, 25: 16x4x4 bricks of 256 threads (two workgroups per CU in FP64) as
 //       21 and 20; 26: 8x8x4 bricks of 256 threads as 21
+//   27: 24 with stages 2-3 loading the y values of the next stage input
+//       during their last passes (y_early); 28 exists for the mixed path only
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -118,18 +120,22 @@ bool brick_fits_v26(const ewpt_lattice &L);
 hipError_t launch_brick_v26(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
-int brick_variant_count() { return 26; }
+bool brick_fits_v27(const ewpt_lattice &L);
+hipError_t launch_brick_v27(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+int brick_variant_count() { return 27; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 26 && v != 9 && order <= 4;
+    return v >= 1 && v <= 28 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)
 {
     const int v = variant % 100;
-    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23 || v == 24 || v == 26);
+    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23 || v == 24 || v == 26 || v == 27 || v == 28);
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)
@@ -160,6 +166,7 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 24: return brick_fits_v24(L);
     case 25: return brick_fits_v25(L);
     case 26: return brick_fits_v26(L);
+    case 27: return brick_fits_v27(L);
     default: return false;
     }
 }
@@ -195,6 +202,7 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 24: return launch_brick_v24(stage, b, L, P, dt, order, st);
     case 25: return launch_brick_v25(stage, b, L, P, dt, order, st);
     case 26: return launch_brick_v26(stage, b, L, P, dt, order, st);
+    case 27: return launch_brick_v27(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }

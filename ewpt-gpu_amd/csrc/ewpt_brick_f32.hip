@@ -10,6 +10,8 @@
 //   21, 23 (+100*order): 20, 22 with stored stage inputs for stages 2-4
 //   24, 25 (+100*order): 16x4x4 bricks of 256 threads as 21, 20
 //   26 (+100*order): 8x8x4 bricks of 256 threads as 21
+//   27 (+100*order): 16x4x4 as 24 with the next input's y loaded early
+//   28 (+100*order): 32x4x4 as 15 with the next input's y loaded early
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -34,6 +36,8 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 24: return brick_fits<16, 4, 4>(L);
     case 25: return brick_fits<16, 4, 4>(L);
     case 26: return brick_fits<8, 8, 4>(L);
+    case 27: return brick_fits<16, 4, 4>(L);
+    case 28: return brick_fits<32, 4, 4>(L);
     default: return false;
     }
 }
@@ -60,6 +64,8 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 24: return launch_tile<16, 4, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
     case 25: return launch_tile<16, 4, 4, 1004432, float>(stage, b, L, P, dt, 1, order, st);
     case 26: return launch_tile<8, 8, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 27: return launch_tile<16, 4, 4, 11704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 28: return launch_tile<32, 4, 4, 10704432, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }
