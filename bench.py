@@ -32,7 +32,7 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7}
+STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:

@@ -281,6 +281,10 @@ __host__ __device__ constexpr bool is_late_channel(int ch) { return (ch >= 16 &&
 // PF / 10000000 = 1: stages 2-3 that write the next stage input load the y
 // values it needs during their last passes instead of after them.
 __host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10 == 1; }
+// PF / 100000000 = 1 (FP64): stage 4 loads y, k1..k3 for the partial sums of
+// channels 16..35 one channel per pass during its first 20 passes instead of
+// all before the first pass.
+__host__ __device__ constexpr bool spread_update(int PF) { return (PF / 100000000) % 10 == 1; }
 __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
@@ -322,24 +326,29 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 
     // ---- stage 4: the field channels of y' and the partial sums n of
     // channels 16..35 (parked in LDS until k4 is known) need only centre
-    // values; they are done first, in two batches of loads.
+    // values; they are done first, in two batches of loads (SPREAD: the
+    // partial sums one per pass during the first NK passes instead).
+    constexpr bool SPREAD = S == 4 && spread_update(PF);
+    static_assert(!SPREAD || !MIXED, "the spread update is an FP64 form");
     if constexpr (S == 4) {
+        if constexpr (!SPREAD) {
 #pragma unroll
-        for (int m = 0; m < NK; m++) {
-            if constexpr (MIXED) {
-                // increment without k4, plus the compensation word
-                R dy = ldo(K(b.k1, m), qb) * d6;
-                dy = ldo(K(b.k2, m), qb) * d3 + dy;
-                dy = ldo(K(b.k3, m), qb) * d3 + dy;
-                nstash[m * B::NT + t] = ldo(b.ylo + (long)(16 + m) * b.n3, qb) + dy;
-                nstash[(NK + m) * B::NT + t] = ldo(Y(16 + m), qb);
-            } else {
-                R n = ldo(K(b.k1, m), qb) * d6 + ldo(Y(16 + m), qb);
-                n = ldo(K(b.k2, m), qb) * d3 + n;
-                nstash[m * B::NT + t] = ldo(K(b.k3, m), qb) * d3 + n;
+            for (int m = 0; m < NK; m++) {
+                if constexpr (MIXED) {
+                    // increment without k4, plus the compensation word
+                    R dy = ldo(K(b.k1, m), qb) * d6;
+                    dy = ldo(K(b.k2, m), qb) * d3 + dy;
+                    dy = ldo(K(b.k3, m), qb) * d3 + dy;
+                    nstash[m * B::NT + t] = ldo(b.ylo + (long)(16 + m) * b.n3, qb) + dy;
+                    nstash[(NK + m) * B::NT + t] = ldo(Y(16 + m), qb);
+                } else {
+                    R n = ldo(K(b.k1, m), qb) * d6 + ldo(Y(16 + m), qb);
+                    n = ldo(K(b.k2, m), qb) * d3 + n;
+                    nstash[m * B::NT + t] = ldo(K(b.k3, m), qb) * d3 + n;
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int ch = 0; ch < 16; ch++) {
             const R piv = ldo(Y(ch + 20), qb);
@@ -715,7 +724,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     constexpr bool YE = y_early(PF) && S > 1 && stores_next(PF, S) && !paired_of(PF, S);
     constexpr int YP = 8;
     R ypre[YE ? NCH : 1];
-    if constexpr (!paired_of(PF, S)) {
+    if constexpr (!paired_of(PF, S) && !SPREAD) {
         store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
         lds_barrier();
         auto body = [&](auto pc) {
@@ -728,6 +737,34 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
             consume(pc, lds[pp & 1]);
             pin_all(pc);
+            if constexpr (pp + 1 < NPASS)
+                store(std::integral_constant<int, pp + 1>{}, rr[(pp + 1) % PFS], lds[(pp + 1) & 1]);
+            lds_barrier();
+        };
+        [&]<int... I>(std::integer_sequence<int, I...>) {
+            (body(std::integral_constant<int, I>{}), ...);
+        }(std::make_integer_sequence<int, NPASS>{});
+    } else if constexpr (!paired_of(PF, S)) {
+        // stage 4 with the partial sums spread over the first NK passes
+        store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
+        lds_barrier();
+        auto body = [&](auto pc) {
+            constexpr int pp = decltype(pc)::value;
+            if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
+            if constexpr (pp < NK) {
+                // the partial sum of channel 16+pp: its loads are in flight while
+                // the pass computes
+                const R k1v = ldo(K(b.k1, pp), qb), k2v = ldo(K(b.k2, pp), qb);
+                const R k3v = ldo(K(b.k3, pp), qb), yv = ldo(Y(16 + pp), qb);
+                consume(pc, lds[pp & 1]);
+                pin_all(pc);
+                R n = k1v * d6 + yv;
+                n = k2v * d3 + n;
+                nstash[pp * B::NT + t] = k3v * d3 + n;
+            } else {
+                consume(pc, lds[pp & 1]);
+                pin_all(pc);
+            }
             if constexpr (pp + 1 < NPASS)
                 store(std::integral_constant<int, pp + 1>{}, rr[(pp + 1) % PFS], lds[(pp + 1) & 1]);
             lds_barrier();

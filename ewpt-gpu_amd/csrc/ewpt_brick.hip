@@ -23,6 +23,7 @@
 //       21 and 20; 26: 8x8x4 bricks of 256 threads as 21
 //   27: 24 with stages 2-3 loading the y values of the next stage input
 //       during their last passes (y_early); 28 exists for the mixed path only
+//   29 (FP64): 27 with stage 4's partial-sum loads spread over its passes
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -124,18 +125,22 @@ bool brick_fits_v27(const ewpt_lattice &L);
 hipError_t launch_brick_v27(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
-int brick_variant_count() { return 27; }
+bool brick_fits_v29(const ewpt_lattice &L);
+hipError_t launch_brick_v29(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
+
+int brick_variant_count() { return 29; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 28 && v != 9 && order <= 4;
+    return v >= 1 && v <= 29 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)
 {
     const int v = variant % 100;
-    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23 || v == 24 || v == 26 || v == 27 || v == 28);
+    return variant >= 0 && ((v >= 14 && v <= 19) || v == 21 || v == 23 || v == 24 || v >= 26);
 }
 
 bool brick_supported(int variant, const ewpt_lattice &L)
@@ -167,6 +172,7 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 25: return brick_fits_v25(L);
     case 26: return brick_fits_v26(L);
     case 27: return brick_fits_v27(L);
+    case 29: return brick_fits_v29(L);
     default: return false;
     }
 }
@@ -203,6 +209,7 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 25: return launch_brick_v25(stage, b, L, P, dt, order, st);
     case 26: return launch_brick_v26(stage, b, L, P, dt, order, st);
     case 27: return launch_brick_v27(stage, b, L, P, dt, order, st);
+    case 29: return launch_brick_v29(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }
