@@ -1,6 +1,6 @@
 // ewpt_brick_v29.hip -- stage-kernel variant 29: as 27 (16x4x4 brick, 256
-// threads, late centres, stored stage inputs, early y) with stage 4's update
-// loads spread over its passes.
+// threads, late centres, stored stage inputs, early y) with stage 4 loading
+// the inputs of its partial sums one channel per pass (spread_update).
 #include "ewpt_brick.h"
 
 namespace ewpt {
