@@ -32,7 +32,7 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7}
+STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7, 30: 7}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:
@@ -67,6 +67,8 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # and WRITE_SIZE on this bench: FETCH_SIZE x 2 (what that counter reads for a
 # plain device copy of known size) + WRITE_SIZE x 1.
 MEASURED_TRAFFIC = {
+    ("fp64", 329, 1): 1113, ("fp64", 329, 2): 1471, ("fp64", 329, 3): 1502, ("fp64", 329, 4): 2376,
+    ("mixed", 328, 1): 633, ("mixed", 328, 2): 854, ("mixed", 328, 3): 848, ("mixed", 328, 4): 1361,
     ("fp64", 324, 1): 1154, ("fp64", 324, 2): 1387, ("fp64", 324, 3): 1386, ("fp64", 324, 4): 2407,
     ("fp64", 321, 1): 1113, ("fp64", 321, 2): 1395, ("fp64", 321, 3): 1415, ("fp64", 321, 4): 2349,
     ("fp64", 311, 1): 849, ("fp64", 311, 2): 1630, ("fp64", 311, 3): 2212, ("fp64", 311, 4): 3725,

@@ -44,7 +44,7 @@ struct BatchDesc {
 // registers for the whole loop body).
 template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T, bool PERSIST>
 __global__ void __launch_bounds__(TX *TY *TZ)
-__attribute__((amdgpu_waves_per_eu(std::is_same_v<T, float> && S < 4 ? 4 : 2)))
+__attribute__((amdgpu_waves_per_eu(waves_per_simd(std::is_same_v<T, float>, PF, S))))
 stage_brick_batch(const BatchDesc<T> *desc, const int2 *tiles, int ntiles)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -100,7 +100,7 @@ struct Kern {
     static size_t lds(int stage)
     {
         const size_t l13 = 2 * B::BUF * sizeof(T);
-        return stage < 4 ? l13 : l13 + (std::is_same_v<T, float> ? 2 : 1) * NK * B::NT * sizeof(T);
+        return stage < 4 ? l13 : l13 + (std::is_same_v<T, float> && !lean_stash(PF) ? 2 : 1) * NK * B::NT * sizeof(T);
     }
     template <bool BETA, int S, bool PS>
     static const void *fn()

@@ -286,6 +286,11 @@ __host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10
 // all before the first pass.
 __host__ __device__ constexpr bool spread_update(int PF) { return (PF / 100000000) % 10 == 1; }
 __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
+// PF / 1000000000 = 1 (mixed): stage 4 parks only the increment in LDS and
+// reads the state's high word again for the final TwoSum, which halves its
+// LDS and lets two workgroups share a CU (four waves per SIMD).
+__host__ __device__ constexpr bool lean_stash(int PF) { return (PF / 1000000000) % 10 == 1; }
+__host__ __device__ constexpr int waves_per_simd(bool f32, int PF, int S) { return f32 && (S < 4 || lean_stash(PF)) ? 4 : 2; }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
 // PF: prefetch profile (pf_of_stage).  Two waves per SIMD are requested for
@@ -340,7 +345,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                     dy = ldo(K(b.k2, m), qb) * d3 + dy;
                     dy = ldo(K(b.k3, m), qb) * d3 + dy;
                     nstash[m * B::NT + t] = ldo(b.ylo + (long)(16 + m) * b.n3, qb) + dy;
-                    nstash[(NK + m) * B::NT + t] = ldo(Y(16 + m), qb);
+                    if constexpr (!lean_stash(PF)) nstash[(NK + m) * B::NT + t] = ldo(Y(16 + m), qb);
                 } else {
                     R n = ldo(K(b.k1, m), qb) * d6 + ldo(Y(16 + m), qb);
                     n = ldo(K(b.k2, m), qb) * d3 + n;
@@ -833,7 +838,8 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 #pragma unroll
         for (int m = 0; m < NK; m++) {
             R nh, nl;
-            two_sum(nstash[(NK + m) * B::NT + t], o[m] * d6 + nstash[m * B::NT + t], nh, nl);
+            const R hi = lean_stash(PF) ? R(ldo(Y(16 + m), qb)) : nstash[(NK + m) * B::NT + t];
+            two_sum(hi, o[m] * d6 + nstash[m * B::NT + t], nh, nl);
             stn(b.ynew + (long)(16 + m) * b.n3, qb, nh);
             stn(b.ylo_new + (long)(16 + m) * b.n3, qb, nl);
         }
@@ -845,7 +851,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 
 template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T = double>
 __global__ void __launch_bounds__(TX *TY *TZ)
-__attribute__((amdgpu_waves_per_eu(std::is_same_v<T, float> && S < 4 ? 4 : 2)))
+__attribute__((amdgpu_waves_per_eu(waves_per_simd(std::is_same_v<T, float>, PF, S))))
 stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -874,7 +880,8 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
     const dim3 grid(g.nbx * g.nby * g.nbz), block(B::NT);
     const size_t lds13 = (paired_of(PF, 1) ? 4 : 2) * B::BUF * sizeof(T);
     // stage 4 (never paired) parks n (FP64) or the increment and the high word (mixed)
-    const size_t lds4 = 2 * B::BUF * sizeof(T) + (std::is_same_v<T, float> ? 2 : 1) * NK * B::NT * sizeof(T);
+    const size_t lds4 =
+        2 * B::BUF * sizeof(T) + (std::is_same_v<T, float> && !lean_stash(PF) ? 2 : 1) * NK * B::NT * sizeof(T);
     static bool attr_set = false;   // the kernels may need more than 64 KB of LDS
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4, T>),

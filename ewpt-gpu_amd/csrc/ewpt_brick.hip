@@ -24,6 +24,7 @@
 //   27: 24 with stages 2-3 loading the y values of the next stage input
 //       during their last passes (y_early); 28 exists for the mixed path only
 //   29 (FP64): 27 with stage 4's partial-sum loads spread over its passes
+//   30 exists for the mixed path only
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -134,7 +135,7 @@ int brick_variant_count() { return 29; }
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 29 && v != 9 && order <= 4;
+    return v >= 1 && v <= 30 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)

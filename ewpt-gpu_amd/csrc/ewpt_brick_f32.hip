@@ -12,6 +12,7 @@
 //   26 (+100*order): 8x8x4 bricks of 256 threads as 21
 //   27 (+100*order): 16x4x4 as 24 with the next input's y loaded early
 //   28 (+100*order): 32x4x4 as 15 with the next input's y loaded early
+//   30 (+100*order): 28 with the lean stage-4 stash (two workgroups per CU)
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -38,6 +39,7 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 26: return brick_fits<8, 8, 4>(L);
     case 27: return brick_fits<16, 4, 4>(L);
     case 28: return brick_fits<32, 4, 4>(L);
+    case 30: return brick_fits<32, 4, 4>(L);
     default: return false;
     }
 }
@@ -66,6 +68,7 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 26: return launch_tile<8, 8, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
     case 27: return launch_tile<16, 4, 4, 11704432, float>(stage, b, L, P, dt, 1, order, st);
     case 28: return launch_tile<32, 4, 4, 10704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 30: return launch_tile<32, 4, 4, 1010704432, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -251,10 +251,10 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
  *   0 = automatic: the first of 329, 327, 324, 321, 320, 311, 310, 210,
- *       211, 2 that tiles the lattice, else 9 (mixed: 328, 315, 215, 311,
- *       211, 310, 210); the stored-stage-input variants (329, 327, 328, 324,
- *       321, 315, 215) only when their buffer fits in device memory and not
- *       on slab contexts
+ *       211, 2 that tiles the lattice, else 9 (mixed: 330, 328, 315, 215,
+ *       311, 211, 310, 210); the stored-stage-input variants (329, 327, 330,
+ *       328, 324, 321, 315, 215) only when their buffer fits in device
+ *       memory and not on slab contexts
  *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
  *          5 = 16x8x4 pf1 linear order, 6 = 16x4x4 pf1, 7 = 8x8x4 pf2,
@@ -271,10 +271,11 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *          27 = 24 with stages 2-3 loading the y values of the next stage
  *          input during their last passes; 28 (mixed) = 15 with the same;
  *          29 (FP64) = 27 with stage 4 loading the update's partial-sum
- *          inputs one channel per pass.
- *          14..19, 21, 23, 24, 26..29 hold one more state-sized buffer
+ *          inputs one channel per pass; 30 (mixed) = 28 with stage 4 keeping
+ *          only the increment in LDS (two workgroups per CU).
+ *          14..19, 21, 23, 24, 26..30 hold one more state-sized buffer
  *          (allocated when first selected).  All but 5 use the XCD-aware
- *          block order.  The mixed path takes 10..28.
+ *          block order.  The mixed path takes 10..28 and 30.
  *   9 = gather kernel (one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
