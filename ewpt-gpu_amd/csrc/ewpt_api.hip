@@ -1101,7 +1101,7 @@ void drop_plan(ewpt_batch *b)
 // candidate variants in preference order (measured single-lattice ranking)
 std::vector<int> batch_candidates(bool f32)
 {
-    std::vector<int> v = f32 ? std::vector<int>{315, 215, 311, 211, 411, 310, 210, 15, 11, 10}
+    std::vector<int> v = f32 ? std::vector<int>{330, 328, 315, 215, 311, 211, 411, 310, 210, 15, 11, 10}
                              : std::vector<int>{311, 310, 211, 210, 303, 203, 11, 10, 3, 2};
     const size_t n = v.size();
     for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms

@@ -312,7 +312,7 @@ int ewpt_batch_destroy(ewpt_batch *b);
 const char *ewpt_batch_last_error(ewpt_batch *b);
 /* Brick variant for every lattice (as ewpt_set_variant: 10/11 (+100*order)
  * for both precisions, also 2/3 for FP64 and the stored-stage-input forms
- * 14/15 for mixed, which allocate each context's stage-input buffer), plus
+ * 14/15/28/30 for mixed, which allocate each context's stage-input buffer), plus
  * 1000 for the persistent launch form (a resident grid striding through the
  * bricks instead of one workgroup per brick); 0 = first that tiles all. */
 int ewpt_batch_set_variant(ewpt_batch *b, int variant);
