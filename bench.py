@@ -67,7 +67,8 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # and WRITE_SIZE on this bench: FETCH_SIZE x 2 (what that counter reads for a
 # plain device copy of known size) + WRITE_SIZE x 1.
 MEASURED_TRAFFIC = {
-    ("fp64", 329, 1): 1113, ("fp64", 329, 2): 1471, ("fp64", 329, 3): 1502, ("fp64", 329, 4): 2376,
+    ("fp64", 329, 1): 1107, ("fp64", 329, 2): 1521, ("fp64", 329, 3): 1498, ("fp64", 329, 4): 2401,
+    ("mixed", 330, 1): 626, ("mixed", 330, 2): 851, ("mixed", 330, 3): 855, ("mixed", 330, 4): 1495,
     ("mixed", 328, 1): 633, ("mixed", 328, 2): 854, ("mixed", 328, 3): 848, ("mixed", 328, 4): 1361,
     ("fp64", 324, 1): 1154, ("fp64", 324, 2): 1387, ("fp64", 324, 3): 1386, ("fp64", 324, 4): 2407,
     ("fp64", 321, 1): 1113, ("fp64", 321, 2): 1395, ("fp64", 321, 3): 1415, ("fp64", 321, 4): 2349,
