@@ -32,7 +32,7 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7, 30: 7}
+STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7, 30: 7, 31: 7}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:
@@ -68,6 +68,7 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # plain device copy of known size) + WRITE_SIZE x 1.
 MEASURED_TRAFFIC = {
     ("fp64", 329, 1): 1107, ("fp64", 329, 2): 1521, ("fp64", 329, 3): 1498, ("fp64", 329, 4): 2401,
+    ("mixed", 331, 1): 632, ("mixed", 331, 2): 850, ("mixed", 331, 3): 845, ("mixed", 331, 4): 1495,
     ("mixed", 330, 1): 626, ("mixed", 330, 2): 851, ("mixed", 330, 3): 855, ("mixed", 330, 4): 1495,
     ("mixed", 328, 1): 633, ("mixed", 328, 2): 854, ("mixed", 328, 3): 848, ("mixed", 328, 4): 1361,
     ("fp64", 324, 1): 1154, ("fp64", 324, 2): 1387, ("fp64", 324, 3): 1386, ("fp64", 324, 4): 2407,
@@ -461,8 +462,9 @@ def main(argv=None) -> int:
         return batch_bench(a)
 
     if a.gpus != 1:
-        print(f"bench.py: this build runs the hot path on one GPU; --gpus {a.gpus} is not supported "
-              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        print(f"bench.py: the benchmark runs the RK4 step on one GPU; --gpus {a.gpus} is not supported "
+              "(z-slab sharding runs its one-GPU share with --slab; transfers between GPUs are not run)",
+              file=sys.stderr)
         return 2
 
     import ewpt

@@ -13,6 +13,7 @@
 //   27 (+100*order): 16x4x4 as 24 with the next input's y loaded early
 //   28 (+100*order): 32x4x4 as 15 with the next input's y loaded early
 //   30 (+100*order): 28 with the lean stage-4 stash (two workgroups per CU)
+//   31 (+100*order): 30 with prefetch depth 4/4/4/3 (stages 3-4 read one input array)
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -40,6 +41,7 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 27: return brick_fits<16, 4, 4>(L);
     case 28: return brick_fits<32, 4, 4>(L);
     case 30: return brick_fits<32, 4, 4>(L);
+    case 31: return brick_fits<32, 4, 4>(L);
     default: return false;
     }
 }
@@ -69,6 +71,7 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 27: return launch_tile<16, 4, 4, 11704432, float>(stage, b, L, P, dt, 1, order, st);
     case 28: return launch_tile<32, 4, 4, 10704432, float>(stage, b, L, P, dt, 1, order, st);
     case 30: return launch_tile<32, 4, 4, 1010704432, float>(stage, b, L, P, dt, 1, order, st);
+    case 31: return launch_tile<32, 4, 4, 1010704443, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3
         v[3] = rY * rY;
         const double cY = s.c[CH_SIG] - (s.dY[0][0] + s.dY[1][1] + s.dY[2][2]);
         v[7] = cY * cY;
-        // |E^a|^2 and |E^Y|^2, the scale the residuals are judged against
+        // |E^a|^2 and |E^Y|^2, the scale the residuals are measured against
 #pragma unroll
         for (int q = 0; q < 9; q++) v[8] += s.c[CH_E + q] * s.c[CH_E + q];
         v[9] = s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2];

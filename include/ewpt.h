@@ -35,7 +35,7 @@ extern "C" {
 #define EWPT_ERR_OOM          -3  /* device allocation failed */
 #define EWPT_ERR_NONFINITE    -4  /* a reduction met NaN/Inf in the state */
 #define EWPT_ERR_STATE        -5  /* call made in the wrong order (no state) */
-#define EWPT_ERR_UNSUPPORTED  -6  /* valid request this build cannot serve */
+#define EWPT_ERR_UNSUPPORTED  -6  /* valid request the library cannot serve */
 
 /* precision of a context */
 #define EWPT_FP64   0   /* FP64 storage and arithmetic */
@@ -251,9 +251,9 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
  *   0 = automatic: the first of 329, 327, 324, 321, 320, 311, 310, 210,
- *       211, 2 that tiles the lattice, else 9 (mixed: 330, 328, 315, 215,
- *       311, 211, 310, 210); the stored-stage-input variants (329, 327, 330,
- *       328, 324, 321, 315, 215) only when their buffer fits in device
+ *       211, 2 that tiles the lattice, else 9 (mixed: 331, 330, 328, 315,
+ *       215, 311, 211, 310, 210); the stored-stage-input variants (329, 327,
+ *       331, 330, 328, 324, 321, 315, 215) only when their buffer fits in device
  *       memory and not on slab contexts
  *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
@@ -272,10 +272,11 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *          input during their last passes; 28 (mixed) = 15 with the same;
  *          29 (FP64) = 27 with stage 4 loading the update's partial-sum
  *          inputs one channel per pass; 30 (mixed) = 28 with stage 4 keeping
- *          only the increment in LDS (two workgroups per CU).
- *          14..19, 21, 23, 24, 26..30 hold one more state-sized buffer
+ *          only the increment in LDS (two workgroups per CU); 31 (mixed) =
+ *          30 with prefetch depth 4/4/4/3.
+ *          14..19, 21, 23, 24, 26..31 hold one more state-sized buffer
  *          (allocated when first selected).  All but 5 use the XCD-aware
- *          block order.  The mixed path takes 10..28 and 30.
+ *          block order.  The mixed path takes 10..28, 30 and 31.
  *   9 = gather kernel (one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
@@ -312,7 +313,7 @@ int ewpt_batch_destroy(ewpt_batch *b);
 const char *ewpt_batch_last_error(ewpt_batch *b);
 /* Brick variant for every lattice (as ewpt_set_variant: 10/11 (+100*order)
  * for both precisions, also 2/3 for FP64 and the stored-stage-input forms
- * 14/15/28/30 for mixed, which allocate each context's stage-input buffer), plus
+ * 14/15/28/30/31 for mixed, which allocate each context's stage-input buffer), plus
  * 1000 for the persistent launch form (a resident grid striding through the
  * bricks instead of one workgroup per brick); 0 = first that tiles all. */
 int ewpt_batch_set_variant(ewpt_batch *b, int variant);
