@@ -176,7 +176,9 @@ int ewpt_slab_buffers(ewpt_ctx *ctx, void **send_lo, void **send_hi);
 int ewpt_slab_pack(ewpt_ctx *ctx, int what, void *send_lo, void *send_hi);
 int ewpt_slab_unpack(ewpt_ctx *ctx, int what, const void *recv_lo, const void *recv_hi);
 /* One stage launch (1..4) without exchange, asynchronous; stage 4 makes
- * the new state current.  Any context. */
+ * the new state current.  Any context.  Call the stages of a step in order
+ * 1, 2, 3, 4: stage s reads the derivatives (and, with stored stage inputs,
+ * the input array) that stage s-1 wrote. */
 int ewpt_stage(ewpt_ctx *ctx, const ewpt_params *p, int stage);
 /* The context's HIP stream (hipStream_t) and a wait for it. */
 int ewpt_get_stream(ewpt_ctx *ctx, void **stream);
