@@ -6,6 +6,8 @@
 // the RK4 bookkeeping of ewpt_stage.h: the stage input y + c*k is formed on
 // load and stage 4 writes y' directly.  It works for any lattice size >= 4.
 // It also provides the energy and Gauss-law reductions.
+#include <utility>
+
 #include "ewpt_internal.h"
 
 namespace ewpt {
@@ -188,30 +190,49 @@ __device__ void block_reduce(double (&v)[NQ], unsigned minmask, double *part)
 // Energy densities at one site, the reference's compute_* kernels:
 // e[0] PE (main_rk4.jl:1341-1342), e[1] KE (1659), e[2] GE (1498-1501),
 // e[3] EEW (1817-1820), e[4] MEW (1978-1981), e[5] EEY (2139-2140), e[6] MEY (2298).
-__device__ __forceinline__ void energy_terms(const SiteIn<double> &s, const Phys &P, double e[7])
+// The terms that need only centre values: e[0], e[1], e[3], e[5].
+__device__ __forceinline__ void energy_centre_terms(const SiteIn<double> &s, const Phys &P, double e[7])
 {
     const double r2 = s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3];
     const double X = r2 - P.vev * P.vev;
     e[0] = P.lam * (X * X);
     e[1] = s.c[CH_PI] * s.c[CH_PI] + s.c[CH_PI + 1] * s.c[CH_PI + 1] + s.c[CH_PI + 2] * s.c[CH_PI + 2] +
            s.c[CH_PI + 3] * s.c[CH_PI + 3];
+    double ee = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) ee += s.c[CH_E + q] * s.c[CH_E + q];
+    e[3] = 0.5 * ee;
+    e[5] = 0.5 * (s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2]);
+}
+
+// e[2] = sum |D_i phi|^2: centres of phi, W, Y and the phi gradients.
+__device__ __forceinline__ double energy_gradient_term(const SiteIn<double> &s, const Phys &P)
+{
     double D[4][3];
     cov_derivs(s, P.gw, P.gy, D);
     double ge = 0.0;
 #pragma unroll
     for (int a = 0; a < 4; a++) ge += D[a][0] * D[a][0] + D[a][1] * D[a][1] + D[a][2] * D[a][2];
-    e[2] = ge;
+    return ge;
+}
+
+// e[4], e[6]: centres of W and the W, Y gradients.
+__device__ __forceinline__ void energy_magnetic_terms(const SiteIn<double> &s, const Phys &P, double e[7])
+{
     double B[3][3], BYv[3];
     field_strengths(s, P.gw, B, BYv);
-    double ee = 0.0, me = 0.0;
-#pragma unroll
-    for (int q = 0; q < 9; q++) ee += s.c[CH_E + q] * s.c[CH_E + q];
+    double me = 0.0;
 #pragma unroll
     for (int a = 0; a < 3; a++) me += B[a][0] * B[a][0] + B[a][1] * B[a][1] + B[a][2] * B[a][2];
-    e[3] = 0.5 * ee;
     e[4] = 0.5 * me;
-    e[5] = 0.5 * (s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2]);
     e[6] = 0.5 * (BYv[0] * BYv[0] + BYv[1] * BYv[1] + BYv[2] * BYv[2]);
+}
+
+__device__ __forceinline__ void energy_terms(const SiteIn<double> &s, const Phys &P, double e[7])
+{
+    energy_centre_terms(s, P, e);
+    e[2] = energy_gradient_term(s, P);
+    energy_magnetic_terms(s, P, e);
 }
 
 // slots: 0 PE, 1 KE, 2 GE, 3 EEW, 4 MEW, 5 EEY, 6 MEY, 7 sum|phi|, 8 min|phi|,
@@ -238,6 +259,116 @@ __global__ void __launch_bounds__(BX *BY) energy_density(const double *y, long n
         v[9] = (double)bad;
     }
     block_reduce<10>(v, 1u << 8, part);
+}
+
+// The same densities on 16x4x4 bricks of 256 threads: each of the 16
+// stencilled channels (phi, W, Y) is staged once with its 3-deep star halo
+// in LDS and differentiated there, so a value is read from memory once per
+// brick instead of once per tap.  The derivatives use sten_d1 on the same
+// taps as gather_site, so every site's densities equal energy_density's bit
+// for bit; only the grouping of the block sums differs.
+constexpr int EBX = 16, EBY = 4, EBZ = 4, EBT = EBX * EBY * EBZ;
+static_assert(EBT == BX * BY, "block_reduce assumes BX*BY threads");
+constexpr int EHX = EBX + 6, EHY = EBY + 6, EHZ = EBZ + 6, EBOX = EHX * EHY * EHZ;
+
+// Stage channel yc of brick (i0, j0, k0) with its star halo into the tile
+// and return the three first derivatives at thread t's site.  Not inlined:
+// inlined into one body, the 16 passes kept ~500 values live at once.
+__device__ __noinline__ void energy_pass(const __attribute__((address_space(1))) double *yc, int nx, int ny, int nz,
+                                         double inv60dx, int i0, int j0, int k0, int t, double *tile, double *d)
+{
+    // all of this thread's loads in flight at once, then into the tile
+    constexpr int NIT = (EBOX + EBT - 1) / EBT;
+    double val[NIT];
+    bool use[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+        const int e = t + it * EBT;
+        const int lx = e % EHX, ly = (e / EHX) % EHY, lz = e / (EHX * EHY);
+        const int out = (lx < 3 || lx >= EBX + 3) + (ly < 3 || ly >= EBY + 3) + (lz < 3 || lz >= EBZ + 3);
+        use[it] = e < EBOX && out <= 1;   // edges and corners of the box are never read
+        if (use[it]) {
+            const int gi = wrap_idx(i0 + lx - 3, nx), gj = wrap_idx(j0 + ly - 3, ny),
+                      gk = wrap_idx(k0 + lz - 3, nz);
+            val[it] = yc[gi + (long)nx * (gj + (long)ny * gk)];
+        }
+    }
+    __syncthreads();   // the previous channel's reads of the tile are done
+#pragma unroll
+    for (int it = 0; it < NIT; it++)
+        if (use[it]) tile[t + it * EBT] = val[it];
+    __syncthreads();
+    const int tx = t % EBX, ty = (t / EBX) % EBY, tz = t / (EBX * EBY);
+    const int c0 = (tx + 3) + EHX * ((ty + 3) + EHY * (tz + 3));
+    constexpr int sx = 1, sy = EHX, sz = EHX * EHY;
+    d[0] = sten_d1(tile[c0 - 3 * sx], tile[c0 - 2 * sx], tile[c0 - sx], tile[c0 + sx], tile[c0 + 2 * sx],
+                   tile[c0 + 3 * sx], inv60dx);
+    d[1] = sten_d1(tile[c0 - 3 * sy], tile[c0 - 2 * sy], tile[c0 - sy], tile[c0 + sy], tile[c0 + 2 * sy],
+                   tile[c0 + 3 * sy], inv60dx);
+    d[2] = sten_d1(tile[c0 - 3 * sz], tile[c0 - 2 * sz], tile[c0 - sz], tile[c0 + sz], tile[c0 + 2 * sz],
+                   tile[c0 + 3 * sz], inv60dx);
+}
+
+template <int C>
+__device__ __forceinline__ void energy_stage_channel(const double *y, long n3, const Geo &g, int i0, int j0, int k0,
+                                                     int t, double *tile, SiteIn<double> &s)
+{
+    // global address space: generic (FLAT) loads would also count against
+    // the LDS wait of the barriers
+    const __attribute__((address_space(1))) double *yc =
+        (const __attribute__((address_space(1))) double *)(y + (long)C * n3);
+    double d[3];
+    energy_pass(yc, g.nx, g.ny, g.nz, g.inv60dx, i0, j0, k0, t, tile, d);
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        if constexpr (C < 4) s.dphi[C][ax] = d[ax];
+        else if constexpr (C < 13) s.dW[(C - 4) / 3][(C - 4) % 3][ax] = d[ax];
+        else s.dY[C - 13][ax] = d[ax];
+    }
+}
+
+template <int... Cs>
+__device__ __forceinline__ void energy_stage_all(const double *y, long n3, const Geo &g, int i0, int j0, int k0,
+                                                 int t, double *tile, SiteIn<double> &s,
+                                                 std::integer_sequence<int, Cs...>)
+{
+    (energy_stage_channel<Cs>(y, n3, g, i0, j0, k0, t, tile, s), ...);
+}
+
+__global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3))) energy_brick(const double *y, long n3, Geo g, Phys P, double *part)
+{
+    __shared__ double tile[EBOX];
+    const int t = threadIdx.x;
+    const int nbx = g.nx / EBX, nby = g.ny / EBY;
+    const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+    const int i0 = bx * EBX, j0 = by * EBY, k0 = bz * EBZ;
+    const int tx = t % EBX, ty = (t / EBX) % EBY, tz = t / (EBX * EBY);
+    const long q = (i0 + tx) + (long)g.nx * ((j0 + ty) + (long)g.ny * (k0 + tz));
+    SiteIn<double> s;
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, INFINITY, 0};
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+        if (c < 16 || c >= CH_PI) s.c[c] = y[(long)c * n3 + q];
+    energy_centre_terms(s, P, v);   // the momenta are not needed after this
+    // phi passes, then the gradient term (the phi gradients die), then W and Y
+    energy_stage_all(y, n3, g, i0, j0, k0, t, tile, s, std::integer_sequence<int, 0, 1, 2, 3>{});
+    v[2] = energy_gradient_term(s, P);
+    energy_stage_all(y, n3, g, i0, j0, k0, t, tile, s,
+                     std::integer_sequence<int, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>{});
+    energy_magnetic_terms(s, P, v);
+    const double m = sqrt(s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3]);
+    v[7] = m;
+    v[8] = m;
+    int bad = 0;
+#pragma unroll
+    for (int qq = 0; qq < 8; qq++) bad += !isfinite(v[qq]);
+    v[9] = (double)bad;
+    block_reduce<10>(v, 1u << 8, part);
+}
+
+bool energy_brick_fits(const ewpt_lattice &L)
+{
+    return L.nx % EBX == 0 && L.ny % EBY == 0 && L.nz % EBZ == 0 && L.nx >= EBX && L.ny >= EBY && L.nz >= EBZ;
 }
 
 // The reference's y-slice snapshot (slices_arr, main_rk4.jl:5700-5930,
@@ -428,8 +559,14 @@ hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P
 {
     const Geo g = make_geo(L);
     const long n3 = (long)L.nx * L.ny * L.nz;
-    energy_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
-    finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 1u << 8, res);
+    if (energy_brick_fits(L)) {
+        const long nb = n3 / EBT;   // <= reduce_blocks(L), the partials' capacity
+        energy_brick<<<dim3((unsigned)nb), dim3(EBT), 0, st>>>(y, n3, g, P, part);
+        finish_reduce<<<10, 256, 0, st>>>(part, nb, 1u << 8, res);
+    } else {
+        energy_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
+        finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 1u << 8, res);
+    }
     return hipGetLastError();
 }
 

@@ -70,8 +70,10 @@ def test_200_steps_32():
     np.testing.assert_allclose(gs, cref.gauss(r, p, DX), rtol=1e-9, atol=1e-14)
 
 
-def test_energies_gauss():
-    shape = (16, 24, 20)
+@pytest.mark.parametrize("shape", [(16, 24, 20), (20, 12, 12)])
+def test_energies_gauss(shape):
+    """(16, 24, 20) takes the brick energy kernel, (20, 12, 12) (nx not a
+    multiple of 16) the one-site-per-thread gather kernel."""
     p = fields.params()
     s = fields.smooth_state(shape, seed=6)
     with ewpt.Lattice(shape, DX) as lat:
