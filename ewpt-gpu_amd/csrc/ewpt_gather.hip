@@ -273,7 +273,9 @@ constexpr int EHX = EBX + 6, EHY = EBY + 6, EHZ = EBZ + 6, EBOX = EHX * EHY * EH
 
 // Stage channel yc of brick (i0, j0, k0) with its star halo into the tile
 // and return the three first derivatives at thread t's site.  Not inlined:
-// inlined into one body, the 16 passes kept ~500 values live at once.
+// inlined into one body, the 16 passes kept ~500 values live at once.  One
+// copy per calling kernel (Tag): a shared copy cost the energy kernel spills.
+template <int Tag>
 __device__ __noinline__ void energy_pass(const __attribute__((address_space(1))) double *yc, int nx, int ny, int nz,
                                          double inv60dx, int i0, int j0, int k0, int t, double *tile, double *d)
 {
@@ -309,7 +311,7 @@ __device__ __noinline__ void energy_pass(const __attribute__((address_space(1)))
                    tile[c0 + 3 * sz], inv60dx);
 }
 
-template <int C>
+template <int Tag, int C>
 __device__ __forceinline__ void energy_stage_channel(const double *y, long n3, const Geo &g, int i0, int j0, int k0,
                                                      int t, double *tile, SiteIn<double> &s)
 {
@@ -318,7 +320,7 @@ __device__ __forceinline__ void energy_stage_channel(const double *y, long n3, c
     const __attribute__((address_space(1))) double *yc =
         (const __attribute__((address_space(1))) double *)(y + (long)C * n3);
     double d[3];
-    energy_pass(yc, g.nx, g.ny, g.nz, g.inv60dx, i0, j0, k0, t, tile, d);
+    energy_pass<Tag>(yc, g.nx, g.ny, g.nz, g.inv60dx, i0, j0, k0, t, tile, d);
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
         if constexpr (C < 4) s.dphi[C][ax] = d[ax];
@@ -327,12 +329,12 @@ __device__ __forceinline__ void energy_stage_channel(const double *y, long n3, c
     }
 }
 
-template <int... Cs>
+template <int Tag, int... Cs>
 __device__ __forceinline__ void energy_stage_all(const double *y, long n3, const Geo &g, int i0, int j0, int k0,
                                                  int t, double *tile, SiteIn<double> &s,
                                                  std::integer_sequence<int, Cs...>)
 {
-    (energy_stage_channel<Cs>(y, n3, g, i0, j0, k0, t, tile, s), ...);
+    (energy_stage_channel<Tag, Cs>(y, n3, g, i0, j0, k0, t, tile, s), ...);
 }
 
 __global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3))) energy_brick(const double *y, long n3, Geo g, Phys P, double *part)
@@ -351,10 +353,10 @@ __global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3))) e
         if (c < 16 || c >= CH_PI) s.c[c] = y[(long)c * n3 + q];
     energy_centre_terms(s, P, v);   // the momenta are not needed after this
     // phi passes, then the gradient term (the phi gradients die), then W and Y
-    energy_stage_all(y, n3, g, i0, j0, k0, t, tile, s, std::integer_sequence<int, 0, 1, 2, 3>{});
+    energy_stage_all<0>(y, n3, g, i0, j0, k0, t, tile, s, std::integer_sequence<int, 0, 1, 2, 3>{});
     v[2] = energy_gradient_term(s, P);
-    energy_stage_all(y, n3, g, i0, j0, k0, t, tile, s,
-                     std::integer_sequence<int, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>{});
+    energy_stage_all<0>(y, n3, g, i0, j0, k0, t, tile, s,
+                        std::integer_sequence<int, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>{});
     energy_magnetic_terms(s, P, v);
     const double m = sqrt(s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3]);
     v[7] = m;
@@ -494,6 +496,32 @@ __global__ void __launch_bounds__(BX *BY) em_density(const double *y, long n3, G
     block_reduce<4>(v, 0u, part);
 }
 
+// em_density on the bricks of energy_brick (the same staged derivatives,
+// the same em_point: every site's values equal em_density's bit for bit).
+__global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3)))
+em_brick(const double *y, long n3, Geo g, Phys P, double sw, double cw, double *part)
+{
+    __shared__ double tile[EBOX];
+    const int t = threadIdx.x;
+    const int nbx = g.nx / EBX, nby = g.ny / EBY;
+    const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+    const int i0 = bx * EBX, j0 = by * EBY, k0 = bz * EBZ;
+    const int tx = t % EBX, ty = (t / EBX) % EBY, tz = t / (EBX * EBY);
+    const long q = (i0 + tx) + (long)g.nx * ((j0 + ty) + (long)g.ny * (k0 + tz));
+    SiteIn<double> s;
+#pragma unroll
+    for (int c = 0; c < 16; c++) s.c[c] = y[(long)c * n3 + q];
+    energy_stage_all<1>(y, n3, g, i0, j0, k0, t, tile, s, std::make_integer_sequence<int, 16>{});
+    double Bv[3], B2[3], A[3];
+    em_point(s, P, sw, cw, Bv, B2, A);
+    double v[4];
+    v[0] = Bv[0] * Bv[0] + Bv[1] * Bv[1] + Bv[2] * Bv[2];
+    v[1] = B2[0] * B2[0] + B2[1] * B2[1] + B2[2] * B2[2];
+    v[2] = B2[0] * A[0] + B2[1] * A[1] + B2[2] * A[2];
+    v[3] = (double)(!isfinite(v[0]) + !isfinite(v[1]) + !isfinite(v[2]));
+    block_reduce<4>(v, 0u, part);
+}
+
 // Sum (or min) nblk partials of slot q in a fixed order; one block per slot.
 __global__ void __launch_bounds__(256) finish_reduce(const double *part, long nblk, unsigned minmask, double *res)
 {
@@ -604,8 +632,14 @@ hipError_t launch_em_energies(const double *y, const ewpt_lattice &L, const Phys
 {
     const Geo g = make_geo(L);
     const long n3 = (long)L.nx * L.ny * L.nz;
-    em_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
-    finish_reduce<<<4, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    if (energy_brick_fits(L)) {
+        const long nb = n3 / EBT;   // <= reduce_blocks(L), the partials' capacity
+        em_brick<<<dim3((unsigned)nb), dim3(EBT), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
+        finish_reduce<<<4, 256, 0, st>>>(part, nb, 0u, res);
+    } else {
+        em_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
+        finish_reduce<<<4, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    }
     return hipGetLastError();
 }
 

@@ -21,17 +21,19 @@ DX = 0.25
 SHAPE = (32, 24, 16)
 
 
-def _state(seed):
-    s = fields.noise_state(SHAPE, seed=seed)
+def _state(seed, shape=SHAPE):
+    s = fields.noise_state(shape, seed=seed)
     s[3] += 1.0          # keep |phi| away from 0 so B_2 is well scaled
     return s
 
 
-def test_em_fields_and_energies_match_oracle():
-    s = _state(21)
+@pytest.mark.parametrize("shape", [SHAPE, (20, 12, 12)])
+def test_em_fields_and_energies_match_oracle(shape):
+    """SHAPE takes the brick reductions, (20, 12, 12) the gather kernels."""
+    s = _state(21, shape)
     p = fields.params()
-    print(f"em fields shape={SHAPE}", flush=True)
-    with ewpt.Lattice(SHAPE, DX) as lat:
+    print(f"em fields shape={shape}", flush=True)
+    with ewpt.Lattice(shape, DX) as lat:
         lat.set_state(s)
         got = lat.em_fields(p, TH)
         e = lat.em_energies(p, TH)
