@@ -275,7 +275,9 @@ constexpr int EHX = EBX + 6, EHY = EBY + 6, EHZ = EBZ + 6, EBOX = EHX * EHY * EH
 // and return the three first derivatives at thread t's site.  Not inlined:
 // inlined into one body, the 16 passes kept ~500 values live at once.  One
 // copy per calling kernel (Tag): a shared copy cost the energy kernel spills.
-template <int Tag>
+// DIR >= 0: only the faces across axis DIR are staged and only d[DIR] is
+// formed (the one-direction derivatives of the Gauss residuals).
+template <int Tag, int DIR = -1>
 __device__ __noinline__ void energy_pass(const __attribute__((address_space(1))) double *yc, int nx, int ny, int nz,
                                          double inv60dx, int i0, int j0, int k0, int t, double *tile, double *d)
 {
@@ -288,7 +290,10 @@ __device__ __noinline__ void energy_pass(const __attribute__((address_space(1)))
         const int e = t + it * EBT;
         const int lx = e % EHX, ly = (e / EHX) % EHY, lz = e / (EHX * EHY);
         const int out = (lx < 3 || lx >= EBX + 3) + (ly < 3 || ly >= EBY + 3) + (lz < 3 || lz >= EBZ + 3);
-        use[it] = e < EBOX && out <= 1;   // edges and corners of the box are never read
+        const bool face_ok = DIR < 0 || (DIR == 0 ? !(ly < 3 || ly >= EBY + 3) && !(lz < 3 || lz >= EBZ + 3)
+                                       : DIR == 1 ? !(lx < 3 || lx >= EBX + 3) && !(lz < 3 || lz >= EBZ + 3)
+                                                  : !(lx < 3 || lx >= EBX + 3) && !(ly < 3 || ly >= EBY + 3));
+        use[it] = e < EBOX && out <= 1 && face_ok;   // edges and corners of the box are never read
         if (use[it]) {
             const int gi = wrap_idx(i0 + lx - 3, nx), gj = wrap_idx(j0 + ly - 3, ny),
                       gk = wrap_idx(k0 + lz - 3, nz);
@@ -303,12 +308,15 @@ __device__ __noinline__ void energy_pass(const __attribute__((address_space(1)))
     const int tx = t % EBX, ty = (t / EBX) % EBY, tz = t / (EBX * EBY);
     const int c0 = (tx + 3) + EHX * ((ty + 3) + EHY * (tz + 3));
     constexpr int sx = 1, sy = EHX, sz = EHX * EHY;
-    d[0] = sten_d1(tile[c0 - 3 * sx], tile[c0 - 2 * sx], tile[c0 - sx], tile[c0 + sx], tile[c0 + 2 * sx],
-                   tile[c0 + 3 * sx], inv60dx);
-    d[1] = sten_d1(tile[c0 - 3 * sy], tile[c0 - 2 * sy], tile[c0 - sy], tile[c0 + sy], tile[c0 + 2 * sy],
-                   tile[c0 + 3 * sy], inv60dx);
-    d[2] = sten_d1(tile[c0 - 3 * sz], tile[c0 - 2 * sz], tile[c0 - sz], tile[c0 + sz], tile[c0 + 2 * sz],
-                   tile[c0 + 3 * sz], inv60dx);
+    if constexpr (DIR < 0 || DIR == 0)
+        d[0] = sten_d1(tile[c0 - 3 * sx], tile[c0 - 2 * sx], tile[c0 - sx], tile[c0 + sx], tile[c0 + 2 * sx],
+                       tile[c0 + 3 * sx], inv60dx);
+    if constexpr (DIR < 0 || DIR == 1)
+        d[1] = sten_d1(tile[c0 - 3 * sy], tile[c0 - 2 * sy], tile[c0 - sy], tile[c0 + sy], tile[c0 + 2 * sy],
+                       tile[c0 + 3 * sy], inv60dx);
+    if constexpr (DIR < 0 || DIR == 2)
+        d[2] = sten_d1(tile[c0 - 3 * sz], tile[c0 - 2 * sz], tile[c0 - sz], tile[c0 + sz], tile[c0 + 2 * sz],
+                       tile[c0 + 3 * sz], inv60dx);
 }
 
 template <int Tag, int C>
@@ -405,6 +413,37 @@ __global__ void __launch_bounds__(256) y_slice_kernel(const double *y, long n3, 
 
 // Gauss residuals (squared), slots 0..7 as documented in ewpt.h; slots 8, 9:
 // sum |E^a|^2, sum |E^Y|^2.
+// Squared Gauss residuals and the E scales at one site (slots as in
+// gauss_density): needs the centres, divE, divEY and the diagonal W, Y
+// derivatives of s.
+__device__ __forceinline__ void gauss_terms(const SiteIn<double> &s, const Phys &P, double v[10])
+{
+    const double p[4] = {s.c[0], s.c[1], s.c[2], s.c[3]};
+    const double pi[4] = {s.c[CH_PI], s.c[CH_PI + 1], s.c[CH_PI + 2], s.c[CH_PI + 3]};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double na = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 3; ii++)
+            na -= xprod(a, s.c[CH_W + ii], s.c[CH_W + 3 + ii], s.c[CH_W + 6 + ii], s.c[CH_E + ii],
+                        s.c[CH_E + 3 + ii], s.c[CH_E + 6 + ii]);
+        const double r = s.divE[a] - P.gw * (na + phiT(a, p, pi));
+        v[a] = r * r;
+        const double dW = s.dW[a][0][0] + s.dW[a][1][1] + s.dW[a][2][2];
+        const double cc = s.c[CH_GAM + a] - dW;
+        v[4 + a] = cc * cc;
+    }
+    const double rY = s.divEY - P.gy * phiTY(p, pi);
+    v[3] = rY * rY;
+    const double cY = s.c[CH_SIG] - (s.dY[0][0] + s.dY[1][1] + s.dY[2][2]);
+    v[7] = cY * cY;
+    // |E^a|^2 and |E^Y|^2, the scale the residuals are measured against
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[8] += s.c[CH_E + q] * s.c[CH_E + q];
+    v[9] = s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2];
+}
+
+// slots: 0-2 G^a, 3 G^Y, 4-6 C^a, 7 C^Y (squared), 8 |E^a|^2, 9 |E^Y|^2
 __global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3, Geo g, Phys P, double *part)
 {
     const int i = blockIdx.x * BX + threadIdx.x;
@@ -417,29 +456,7 @@ __global__ void __launch_bounds__(BX *BY) gauss_density(const double *y, long n3
         b.n3 = n3;
         SiteIn<double> s;
         gather_site<1>(b, g, i, j, k, 0.0, s);
-        const double p[4] = {s.c[0], s.c[1], s.c[2], s.c[3]};
-        const double pi[4] = {s.c[CH_PI], s.c[CH_PI + 1], s.c[CH_PI + 2], s.c[CH_PI + 3]};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            double na = 0.0;
-#pragma unroll
-            for (int ii = 0; ii < 3; ii++)
-                na -= xprod(a, s.c[CH_W + ii], s.c[CH_W + 3 + ii], s.c[CH_W + 6 + ii], s.c[CH_E + ii],
-                            s.c[CH_E + 3 + ii], s.c[CH_E + 6 + ii]);
-            const double r = s.divE[a] - P.gw * (na + phiT(a, p, pi));
-            v[a] = r * r;
-            const double dW = s.dW[a][0][0] + s.dW[a][1][1] + s.dW[a][2][2];
-            const double cc = s.c[CH_GAM + a] - dW;
-            v[4 + a] = cc * cc;
-        }
-        const double rY = s.divEY - P.gy * phiTY(p, pi);
-        v[3] = rY * rY;
-        const double cY = s.c[CH_SIG] - (s.dY[0][0] + s.dY[1][1] + s.dY[2][2]);
-        v[7] = cY * cY;
-        // |E^a|^2 and |E^Y|^2, the scale the residuals are measured against
-#pragma unroll
-        for (int q = 0; q < 9; q++) v[8] += s.c[CH_E + q] * s.c[CH_E + q];
-        v[9] = s.c[CH_EY] * s.c[CH_EY] + s.c[CH_EY + 1] * s.c[CH_EY + 1] + s.c[CH_EY + 2] * s.c[CH_EY + 2];
+        gauss_terms(s, P, v);
     }
     block_reduce<10>(v, 0u, part);
 }
@@ -520,6 +537,69 @@ em_brick(const double *y, long n3, Geo g, Phys P, double sw, double cw, double *
     v[2] = B2[0] * A[0] + B2[1] * A[1] + B2[2] * A[2];
     v[3] = (double)(!isfinite(v[0]) + !isfinite(v[1]) + !isfinite(v[2]));
     block_reduce<4>(v, 0u, part);
+}
+
+// The first derivative along AX of channel C at thread t's site, staged
+// through the brick tile with the faces across AX only.
+template <int Tag, int C, int AX>
+__device__ __forceinline__ double brick_deriv(const double *y, long n3, const Geo &g, int i0, int j0, int k0, int t,
+                                              double *tile)
+{
+    double d[3];
+    energy_pass<Tag, AX>((const __attribute__((address_space(1))) double *)(y + (long)C * n3), g.nx, g.ny, g.nz,
+                         g.inv60dx, i0, j0, k0, t, tile, d);
+    return d[AX];
+}
+
+// gauss_density on the bricks of energy_brick: divergences summed in the
+// order of gather_site, the same gauss_terms, so every site's values equal
+// gauss_density's bit for bit.
+__global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3)))
+gauss_brick(const double *y, long n3, Geo g, Phys P, double *part)
+{
+    __shared__ double tile[EBOX];
+    const int t = threadIdx.x;
+    const int nbx = g.nx / EBX, nby = g.ny / EBY;
+    const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+    const int i0 = bx * EBX, j0 = by * EBY, k0 = bz * EBZ;
+    const int tx = t % EBX, ty = (t / EBX) % EBY, tz = t / (EBX * EBY);
+    const long q = (i0 + tx) + (long)g.nx * ((j0 + ty) + (long)g.ny * (k0 + tz));
+    SiteIn<double> s;
+#pragma unroll
+    for (int c = 0; c < NCH; c++)
+        if (c < CH_Y || c >= CH_GAM) s.c[c] = y[(long)c * n3 + q];
+    // divergences: channel E^a_i (CH_E + 3a + i) along i, added in axis order
+    auto div3 = [&](auto c0) {
+        constexpr int C0 = decltype(c0)::value;
+        double dv = 0.0;
+        dv += brick_deriv<2, C0, 0>(y, n3, g, i0, j0, k0, t, tile);
+        dv += brick_deriv<2, C0 + 1, 1>(y, n3, g, i0, j0, k0, t, tile);
+        dv += brick_deriv<2, C0 + 2, 2>(y, n3, g, i0, j0, k0, t, tile);
+        return dv;
+    };
+    s.divE[0] = div3(std::integral_constant<int, CH_E>{});
+    s.divE[1] = div3(std::integral_constant<int, CH_E + 3>{});
+    s.divE[2] = div3(std::integral_constant<int, CH_E + 6>{});
+    s.divEY = div3(std::integral_constant<int, CH_EY>{});
+    // diagonal derivatives d_i W^a_i (channel CH_W + 3a + i) and d_i Y_i
+    auto diag3 = [&](auto c0, double (&out)[3]) {
+        constexpr int C0 = decltype(c0)::value;
+        out[0] = brick_deriv<2, C0, 0>(y, n3, g, i0, j0, k0, t, tile);
+        out[1] = brick_deriv<2, C0 + 1, 1>(y, n3, g, i0, j0, k0, t, tile);
+        out[2] = brick_deriv<2, C0 + 2, 2>(y, n3, g, i0, j0, k0, t, tile);
+    };
+    double dd[3];
+    diag3(std::integral_constant<int, CH_W>{}, dd);
+    s.dW[0][0][0] = dd[0]; s.dW[0][1][1] = dd[1]; s.dW[0][2][2] = dd[2];
+    diag3(std::integral_constant<int, CH_W + 3>{}, dd);
+    s.dW[1][0][0] = dd[0]; s.dW[1][1][1] = dd[1]; s.dW[1][2][2] = dd[2];
+    diag3(std::integral_constant<int, CH_W + 6>{}, dd);
+    s.dW[2][0][0] = dd[0]; s.dW[2][1][1] = dd[1]; s.dW[2][2][2] = dd[2];
+    diag3(std::integral_constant<int, CH_Y>{}, dd);
+    s.dY[0][0] = dd[0]; s.dY[1][1] = dd[1]; s.dY[2][2] = dd[2];
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    gauss_terms(s, P, v);
+    block_reduce<10>(v, 0u, part);
 }
 
 // Sum (or min) nblk partials of slot q in a fixed order; one block per slot.
@@ -603,8 +683,14 @@ hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, d
 {
     const Geo g = make_geo(L);
     const long n3 = (long)L.nx * L.ny * L.nz;
-    gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
-    finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    if (energy_brick_fits(L)) {
+        const long nb = n3 / EBT;   // <= reduce_blocks(L), the partials' capacity
+        gauss_brick<<<dim3((unsigned)nb), dim3(EBT), 0, st>>>(y, n3, g, P, part);
+        finish_reduce<<<10, 256, 0, st>>>(part, nb, 0u, res);
+    } else {
+        gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
+        finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+    }
     return hipGetLastError();
 }
 
