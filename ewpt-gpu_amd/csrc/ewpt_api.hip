@@ -204,8 +204,8 @@ int push_view(ewpt_ctx *c)
 }
 
 // variant actually launched: AUTO takes the first brick variant that tiles
-// the lattice from the measured ranking (512^3 on MI355X, FP64: 329 145.3,
-// 327 148.7, 324 170.3, 321 177.3, 320 181.2, 311 184.4, 310 187.0 ms/step;
+// the lattice from the measured ranking (512^3 on MI355X, FP64: 332 141.0,
+// 329 145.3, 327 148.7, 324 170.3, 321 177.3, 320 181.2, 311 184.4, 310 187.0 ms/step;
 // mixed: 331 82.7, 330 83.4, 328 85.5, 315 89.4, 215 92.1, 311 96.3, 211
 // 103.2 ms/step), else the gather kernel.  The stored-stage-input variants (329,
 // 331, ...) hold one more state-sized buffer; without room for it (or on a
@@ -218,7 +218,7 @@ int resolved_variant(const ewpt_ctx *c)
             if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported_f32(v, c->lat)) return v;
         return -1;
     }
-    for (int v : {329, 327, 324, 321, 320, 311, 310, 210, 211, 2})
+    for (int v : {332, 329, 327, 324, 321, 320, 311, 310, 210, 211, 2})
         if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported(v, c->lat)) return v;
     return VARIANT_GATHER;
 }

@@ -281,15 +281,24 @@ __host__ __device__ constexpr bool is_late_channel(int ch) { return (ch >= 16 &&
 // PF / 10000000 = 1: stages 2-3 that write the next stage input load the y
 // values it needs during their last passes instead of after them.
 __host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10 == 1; }
-// PF / 100000000 = 1 (FP64): stage 4 loads y, k1..k3 for the partial sums of
-// channels 16..35 one channel per pass during its first 20 passes instead of
-// all before the first pass.
-__host__ __device__ constexpr bool spread_update(int PF) { return (PF / 100000000) % 10 == 1; }
+// PF / 100000000 = 1 or 2 (FP64): stage 4 loads y, k1..k3 for the partial
+// sums of channels 16..35 one channel per pass during its first 20 passes
+// instead of all before the first pass (2: fused_fields as well).
+__host__ __device__ constexpr bool spread_update(int PF)
+{
+    return (PF / 100000000) % 10 == 1 || (PF / 100000000) % 10 == 2;
+}
 __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
 // PF / 1000000000 = 1 (mixed): stage 4 parks only the increment in LDS and
 // reads the state's high word again for the final TwoSum, which halves its
 // LDS and lets two workgroups share a CU (four waves per SIMD).
 __host__ __device__ constexpr bool lean_stash(int PF) { return (PF / 1000000000) % 10 == 1; }
+// PF / 100000000 = 2 (FP64): spread_update as 1, and stage 4 updates the
+// field channels of y' in the same passes (field channel ch with the partial
+// sum of channel ch + 20, its momentum) from the same loads of the momentum
+// y and k's: a separate field loop at the start loaded those 64 channels a
+// second time, after L2 had evicted them.
+__host__ __device__ constexpr bool fused_fields(int PF) { return (PF / 100000000) % 10 == 2; }
 __host__ __device__ constexpr int waves_per_simd(bool f32, int PF, int S) { return f32 && (S < 4 || lean_stash(PF)) ? 4 : 2; }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
@@ -335,6 +344,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     // partial sums one per pass during the first NK passes instead).
     constexpr bool SPREAD = S == 4 && spread_update(PF);
     static_assert(!SPREAD || !MIXED, "the spread update is an FP64 form");
+    static_assert(S < 4 || !fused_fields(PF) || SPREAD, "the field update is fused into the spread passes");
     if constexpr (S == 4) {
         if constexpr (!SPREAD) {
 #pragma unroll
@@ -355,7 +365,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int ch = 0; ch < 16; ch++) {
+        for (int ch = 0; ch < (fused_fields(PF) ? 0 : 16); ch++) {
             const R piv = ldo(Y(ch + 20), qb);
             const R k2 = piv + ldo(K(b.k1, ch + 4), qb) * half_dt;
             const R k3 = piv + ldo(K(b.k2, ch + 4), qb) * half_dt;
@@ -766,6 +776,19 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 R n = k1v * d6 + yv;
                 n = k2v * d3 + n;
                 nstash[pp * B::NT + t] = k3v * d3 + n;
+                if constexpr (fused_fields(PF) && pp >= 4) {
+                    // field channel pp - 4: its momentum is channel 16 + pp,
+                    // the expressions of the field loop
+                    constexpr int ch = pp - 4;
+                    const R piv = yv;
+                    const R k2 = piv + k1v * half_dt;
+                    const R k3 = piv + k2v * half_dt;
+                    const R k4 = piv + k3v * dt;
+                    R nf = piv * d6 + ldo(Y(ch), qb);
+                    nf = k2 * d3 + nf;
+                    nf = k3 * d3 + nf;
+                    stn(b.ynew + (long)ch * b.n3, qb, k4 * d6 + nf);
+                }
             } else {
                 consume(pc, lds[pp & 1]);
                 pin_all(pc);

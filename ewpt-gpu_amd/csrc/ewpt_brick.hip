@@ -24,6 +24,7 @@
 //   27: 24 with stages 2-3 loading the y values of the next stage input
 //       during their last passes (y_early); 28 exists for the mixed path only
 //   29 (FP64): 27 with stage 4's partial-sum loads spread over its passes
+//   32 (FP64): 29 with the field channels of y' updated in those passes
 //   30 exists for the mixed path only
 #include "ewpt_internal.h"
 
@@ -130,12 +131,15 @@ bool brick_fits_v29(const ewpt_lattice &L);
 hipError_t launch_brick_v29(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                              int order, hipStream_t st);
 
+bool brick_fits_v32(const ewpt_lattice &L);
+hipError_t launch_brick_v32(int stage, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                             int order, hipStream_t st);
 int brick_variant_count() { return 29; }
 
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 31 && v != 9 && order <= 4;
+    return v >= 1 && v <= 32 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)
@@ -174,6 +178,7 @@ bool brick_supported(int variant, const ewpt_lattice &L)
     case 26: return brick_fits_v26(L);
     case 27: return brick_fits_v27(L);
     case 29: return brick_fits_v29(L);
+    case 32: return brick_fits_v32(L);
     default: return false;
     }
 }
@@ -211,6 +216,7 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
     case 26: return launch_brick_v26(stage, b, L, P, dt, order, st);
     case 27: return launch_brick_v27(stage, b, L, P, dt, order, st);
     case 29: return launch_brick_v29(stage, b, L, P, dt, order, st);
+    case 32: return launch_brick_v32(stage, b, L, P, dt, order, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -252,10 +252,10 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 /* Kernel variant of the stage launches.  All variants compute the same
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
- *   0 = automatic: the first of 329, 327, 324, 321, 320, 311, 310, 210,
- *       211, 2 that tiles the lattice, else 9 (mixed: 331, 330, 328, 315,
- *       215, 311, 211, 310, 210); the stored-stage-input variants (329, 327,
- *       331, 330, 328, 324, 321, 315, 215) only when their buffer fits in device
+ *   0 = automatic: the first of 332, 329, 327, 324, 321, 320, 311, 310,
+ *       210, 211, 2 that tiles the lattice, else 9 (mixed: 331, 330, 328, 315,
+ *       215, 311, 211, 310, 210); the stored-stage-input variants (332, 329,
+ *       327, 331, 330, 328, 324, 321, 315, 215) only when their buffer fits in device
  *       memory and not on slab contexts
  *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
@@ -275,8 +275,9 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *          29 (FP64) = 27 with stage 4 loading the update's partial-sum
  *          inputs one channel per pass; 30 (mixed) = 28 with stage 4 keeping
  *          only the increment in LDS (two workgroups per CU); 31 (mixed) =
- *          30 with prefetch depth 4/4/4/3.
- *          14..19, 21, 23, 24, 26..31 hold one more state-sized buffer
+ *          30 with prefetch depth 4/4/4/3; 32 (FP64) = 29 with the field
+ *          channels of y' updated in the spread passes from the same loads.
+ *          14..19, 21, 23, 24, 26..32 hold one more state-sized buffer
  *          (allocated when first selected).  All but 5 use the XCD-aware
  *          block order.  The mixed path takes 10..28, 30 and 31.
  *   9 = gather kernel (one site per thread, any lattice size)

@@ -114,7 +114,7 @@ def test_init_modes_matches_cpu_rebuild():
 @pytest.mark.parametrize("beta", [0.0, 0.2])
 @pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23,
                                      24, 25, 26, 27, 29, 111, 211, 310, 311, 410, 312, 314, 415, 316, 317, 418, 320,
-                                     321, 322, 323, 324, 325, 326, 327, 329])
+                                     321, 322, 323, 324, 325, 326, 327, 329, 32, 332])
 def test_stage_variants_match_oracle(variant, beta):
     shape = (32, 32, 32)
     print(f"variant {variant} beta {beta} shape={shape}", flush=True)
@@ -174,7 +174,7 @@ def test_stored_stage_inputs_match_partner(stored, partner, precision):
     assert d <= 64 * eps * np.abs(out[1]).max(), d
 
 
-@pytest.mark.parametrize("precision,form,base", [(ewpt.EWPT_FP64, 327, 324), (ewpt.EWPT_FP64, 329, 327),
+@pytest.mark.parametrize("precision,form,base", [(ewpt.EWPT_FP64, 327, 324), (ewpt.EWPT_FP64, 329, 327), (ewpt.EWPT_FP64, 332, 329),
                                                  (ewpt.EWPT_MIXED, 330, 328), (ewpt.EWPT_MIXED, 331, 330)])
 def test_latency_forms_bitwise(precision, form, base):
     """Forms that only move loads earlier or later (the next input's y during
