@@ -32,7 +32,7 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7, 30: 7, 31: 7, 32: 7}
+STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7, 30: 7, 31: 7, 32: 7, 33: 7}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:
@@ -69,6 +69,7 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 MEASURED_TRAFFIC = {
     ("fp64", 332, 1): 1110, ("fp64", 332, 2): 1504, ("fp64", 332, 3): 1500, ("fp64", 332, 4): 1882,
     ("fp64", 329, 1): 1107, ("fp64", 329, 2): 1521, ("fp64", 329, 3): 1498, ("fp64", 329, 4): 2401,
+    ("mixed", 333, 1): 625, ("mixed", 333, 2): 848, ("mixed", 333, 3): 842, ("mixed", 333, 4): 1324,
     ("mixed", 331, 1): 632, ("mixed", 331, 2): 850, ("mixed", 331, 3): 845, ("mixed", 331, 4): 1495,
     ("mixed", 330, 1): 626, ("mixed", 330, 2): 851, ("mixed", 330, 3): 855, ("mixed", 330, 4): 1495,
     ("mixed", 328, 1): 633, ("mixed", 328, 2): 854, ("mixed", 328, 3): 848, ("mixed", 328, 4): 1361,

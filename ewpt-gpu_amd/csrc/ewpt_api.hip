@@ -206,7 +206,7 @@ int push_view(ewpt_ctx *c)
 // variant actually launched: AUTO takes the first brick variant that tiles
 // the lattice from the measured ranking (512^3 on MI355X, FP64: 332 141.0,
 // 329 145.3, 327 148.7, 324 170.3, 321 177.3, 320 181.2, 311 184.4, 310 187.0 ms/step;
-// mixed: 331 82.7, 330 83.4, 328 85.5, 315 89.4, 215 92.1, 311 96.3, 211
+// mixed: 333 78.9, 331 82.7, 330 83.4, 328 85.5, 315 89.4, 215 92.1, 311 96.3, 211
 // 103.2 ms/step), else the gather kernel.  The stored-stage-input variants (329,
 // 331, ...) hold one more state-sized buffer; without room for it (or on a
 // slab) AUTO skips them.
@@ -214,7 +214,7 @@ int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
     if (mixed(c)) {
-        for (int v : {331, 330, 328, 315, 215, 311, 211, 310, 210})
+        for (int v : {333, 331, 330, 328, 315, 215, 311, 211, 310, 210})
             if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported_f32(v, c->lat)) return v;
         return -1;
     }
@@ -1014,7 +1014,7 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
     if (ctx->zg && (variant == VARIANT_GATHER || brick_stored_inputs(variant)))
         return fail(ctx, EWPT_ERR_UNSUPPORTED, "slab contexts take the brick variants without stored stage inputs");
     if (mixed(ctx)) {
-        if (variant % 100 < 10 || variant % 100 > 31) return EWPT_ERR_INVAL;
+        if (variant % 100 < 10 || variant % 100 > 33) return EWPT_ERR_INVAL;
         if (!brick_exists(variant)) return EWPT_ERR_INVAL;
         if (!brick_supported_f32(variant, ctx->lat))
             return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
@@ -1101,7 +1101,7 @@ void drop_plan(ewpt_batch *b)
 // candidate variants in preference order (measured single-lattice ranking)
 std::vector<int> batch_candidates(bool f32)
 {
-    std::vector<int> v = f32 ? std::vector<int>{331, 330, 328, 315, 215, 311, 211, 411, 310, 210, 15, 11, 10}
+    std::vector<int> v = f32 ? std::vector<int>{333, 331, 330, 328, 315, 215, 311, 211, 411, 310, 210, 15, 11, 10}
                              : std::vector<int>{311, 310, 211, 210, 303, 203, 11, 10, 3, 2};
     const size_t n = v.size();
     for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms

@@ -83,7 +83,7 @@ bool tile_of(int v, bool f32, int &tx, int &ty, int &tz)
         // stored stage inputs for stages 2-4 (the mixed path's best single-lattice form)
         switch (v) {
         case 14: tx = 16; ty = 8; tz = 4; return true;
-        case 15: case 28: case 30: case 31: tx = 32; ty = 4; tz = 4; return true;
+        case 15: case 28: case 30: case 31: case 33: tx = 32; ty = 4; tz = 4; return true;
         default: return false;
         }
     }
@@ -202,6 +202,7 @@ auto with_kern(int v, F &&f)
         if (v == 28) return f(Kern<32, 4, 4, 10704432, T>{});
         if (v == 30) return f(Kern<32, 4, 4, 1010704432, T>{});
         if (v == 31) return f(Kern<32, 4, 4, 1010704443, T>{});
+        if (v == 33) return f(Kern<32, 4, 4, 1210704443, T>{});
     }
     if constexpr (std::is_same_v<T, double>) {
         if (v == 2) return f(Kern<8, 8, 8, 1, T>{});

@@ -281,9 +281,10 @@ __host__ __device__ constexpr bool is_late_channel(int ch) { return (ch >= 16 &&
 // PF / 10000000 = 1: stages 2-3 that write the next stage input load the y
 // values it needs during their last passes instead of after them.
 __host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10 == 1; }
-// PF / 100000000 = 1 or 2 (FP64): stage 4 loads y, k1..k3 for the partial
-// sums of channels 16..35 one channel per pass during its first 20 passes
-// instead of all before the first pass (2: fused_fields as well).
+// PF / 100000000 = 1 or 2: stage 4 loads y, k1..k3 for the partial sums of
+// channels 16..35 (mixed: the k's and the compensation word for the
+// increments) one channel per pass during its first 20 passes instead of all
+// before the first pass (2: fused_fields as well).
 __host__ __device__ constexpr bool spread_update(int PF)
 {
     return (PF / 100000000) % 10 == 1 || (PF / 100000000) % 10 == 2;
@@ -293,7 +294,7 @@ __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && 
 // reads the state's high word again for the final TwoSum, which halves its
 // LDS and lets two workgroups share a CU (four waves per SIMD).
 __host__ __device__ constexpr bool lean_stash(int PF) { return (PF / 1000000000) % 10 == 1; }
-// PF / 100000000 = 2 (FP64): spread_update as 1, and stage 4 updates the
+// PF / 100000000 = 2: spread_update as 1, and stage 4 updates the
 // field channels of y' in the same passes (field channel ch with the partial
 // sum of channel ch + 20, its momentum) from the same loads of the momentum
 // y and k's: a separate field loop at the start loaded those 64 channels a
@@ -343,7 +344,6 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     // values; they are done first, in two batches of loads (SPREAD: the
     // partial sums one per pass during the first NK passes instead).
     constexpr bool SPREAD = S == 4 && spread_update(PF);
-    static_assert(!SPREAD || !MIXED, "the spread update is an FP64 form");
     static_assert(S < 4 || !fused_fields(PF) || SPREAD, "the field update is fused into the spread passes");
     if constexpr (S == 4) {
         if constexpr (!SPREAD) {
@@ -766,7 +766,35 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         auto body = [&](auto pc) {
             constexpr int pp = decltype(pc)::value;
             if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
-            if constexpr (pp < NK) {
+            if constexpr (pp < NK && MIXED) {
+                // the increment of channel 16+pp (and with pp >= 4 the field
+                // channel pp - 4, whose momentum it is), as the prologue forms
+                // them; the loads are in flight while the pass computes
+                const R k1v = ldo(K(b.k1, pp), qb), k2v = ldo(K(b.k2, pp), qb);
+                const R k3v = ldo(K(b.k3, pp), qb), ylv = ldo(b.ylo + (long)(16 + pp) * b.n3, qb);
+                consume(pc, lds[pp & 1]);
+                pin_all(pc);
+                R dy = k1v * d6;
+                dy = k2v * d3 + dy;
+                dy = k3v * d3 + dy;
+                nstash[pp * B::NT + t] = ylv + dy;
+                if constexpr (!lean_stash(PF)) nstash[(NK + pp) * B::NT + t] = ldo(Y(16 + pp), qb);
+                if constexpr (fused_fields(PF) && pp >= 4) {
+                    constexpr int ch = pp - 4;
+                    const R piv = ldo(Y(ch + 20), qb);
+                    const R k2 = piv + k1v * half_dt;
+                    const R k3 = piv + k2v * half_dt;
+                    const R k4 = piv + k3v * dt;
+                    R dyf = piv * d6;
+                    dyf = k2 * d3 + dyf;
+                    dyf = k3 * d3 + dyf;
+                    dyf = k4 * d6 + dyf;
+                    R nh, nl;
+                    two_sum(ldo(Y(ch), qb), ldo(b.ylo + (long)ch * b.n3, qb) + dyf, nh, nl);
+                    stn(b.ynew + (long)ch * b.n3, qb, nh);
+                    stn(b.ylo_new + (long)ch * b.n3, qb, nl);
+                }
+            } else if constexpr (pp < NK) {
                 // the partial sum of channel 16+pp: its loads are in flight while
                 // the pass computes
                 const R k1v = ldo(K(b.k1, pp), qb), k2v = ldo(K(b.k2, pp), qb);

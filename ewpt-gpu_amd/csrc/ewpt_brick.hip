@@ -139,7 +139,7 @@ int brick_variant_count() { return 29; }
 bool brick_exists(int variant)
 {
     const int v = variant % 100, order = variant / 100;
-    return v >= 1 && v <= 32 && v != 9 && order <= 4;
+    return v >= 1 && v <= 33 && v != 9 && order <= 4;
 }
 
 bool brick_stored_inputs(int variant)

@@ -14,6 +14,8 @@
 //   28 (+100*order): 32x4x4 as 15 with the next input's y loaded early
 //   30 (+100*order): 28 with the lean stage-4 stash (two workgroups per CU)
 //   31 (+100*order): 30 with prefetch depth 4/4/4/3 (stages 3-4 read one input array)
+//   33 (+100*order): 31 with stage 4's increments formed one channel per pass
+//                    and the field channels updated from the same k loads
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -42,6 +44,7 @@ bool brick_supported_f32(int variant, const ewpt_lattice &L)
     case 28: return brick_fits<32, 4, 4>(L);
     case 30: return brick_fits<32, 4, 4>(L);
     case 31: return brick_fits<32, 4, 4>(L);
+    case 33: return brick_fits<32, 4, 4>(L);
     default: return false;
     }
 }
@@ -72,6 +75,7 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 28: return launch_tile<32, 4, 4, 10704432, float>(stage, b, L, P, dt, 1, order, st);
     case 30: return launch_tile<32, 4, 4, 1010704432, float>(stage, b, L, P, dt, 1, order, st);
     case 31: return launch_tile<32, 4, 4, 1010704443, float>(stage, b, L, P, dt, 1, order, st);
+    case 33: return launch_tile<32, 4, 4, 1210704443, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }

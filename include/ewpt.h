@@ -253,9 +253,9 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
  *   0 = automatic: the first of 332, 329, 327, 324, 321, 320, 311, 310,
- *       210, 211, 2 that tiles the lattice, else 9 (mixed: 331, 330, 328, 315,
- *       215, 311, 211, 310, 210); the stored-stage-input variants (332, 329,
- *       327, 331, 330, 328, 324, 321, 315, 215) only when their buffer fits in device
+ *       210, 211, 2 that tiles the lattice, else 9 (mixed: 333, 331, 330, 328,
+ *       315, 215, 311, 211, 310, 210); the stored-stage-input variants (332, 329,
+ *       327, 333, 331, 330, 328, 324, 321, 315, 215) only when their buffer fits in device
  *       memory and not on slab contexts
  *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
  *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
@@ -276,10 +276,11 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *          inputs one channel per pass; 30 (mixed) = 28 with stage 4 keeping
  *          only the increment in LDS (two workgroups per CU); 31 (mixed) =
  *          30 with prefetch depth 4/4/4/3; 32 (FP64) = 29 with the field
- *          channels of y' updated in the spread passes from the same loads.
- *          14..19, 21, 23, 24, 26..32 hold one more state-sized buffer
+ *          channels of y' updated in the spread passes from the same loads;
+ *          33 (mixed) = 31 with the same spread and fused stage 4.
+ *          14..19, 21, 23, 24, 26..33 hold one more state-sized buffer
  *          (allocated when first selected).  All but 5 use the XCD-aware
- *          block order.  The mixed path takes 10..28, 30 and 31.
+ *          block order.  The mixed path takes 10..28, 30, 31 and 33.
  *   9 = gather kernel (one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
@@ -316,7 +317,7 @@ int ewpt_batch_destroy(ewpt_batch *b);
 const char *ewpt_batch_last_error(ewpt_batch *b);
 /* Brick variant for every lattice (as ewpt_set_variant: 10/11 (+100*order)
  * for both precisions, also 2/3 for FP64 and the stored-stage-input forms
- * 14/15/28/30/31 for mixed, which allocate each context's stage-input buffer), plus
+ * 14/15/28/30/31/33 for mixed, which allocate each context's stage-input buffer), plus
  * 1000 for the persistent launch form (a resident grid striding through the
  * bricks instead of one workgroup per brick); 0 = first that tiles all. */
 int ewpt_batch_set_variant(ewpt_batch *b, int variant);

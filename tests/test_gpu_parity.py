@@ -175,7 +175,8 @@ def test_stored_stage_inputs_match_partner(stored, partner, precision):
 
 
 @pytest.mark.parametrize("precision,form,base", [(ewpt.EWPT_FP64, 327, 324), (ewpt.EWPT_FP64, 329, 327), (ewpt.EWPT_FP64, 332, 329),
-                                                 (ewpt.EWPT_MIXED, 330, 328), (ewpt.EWPT_MIXED, 331, 330)])
+                                                 (ewpt.EWPT_MIXED, 330, 328), (ewpt.EWPT_MIXED, 331, 330),
+                                                 (ewpt.EWPT_MIXED, 333, 331)])
 def test_latency_forms_bitwise(precision, form, base):
     """Forms that only move loads earlier or later (the next input's y during
     the last passes, stage 4's partial-sum inputs one channel per pass, the
@@ -228,8 +229,8 @@ def test_mixed_state_roundtrip():
     assert np.abs(y - s).max() <= 1e-13 * np.abs(s).max()
 
 
-@pytest.mark.parametrize("variant", [0, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 27, 28, 30, 31,
-                                     210, 311, 314, 318, 319, 321, 322, 324, 328, 330, 331, 410])
+@pytest.mark.parametrize("variant", [0, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 27, 28, 30, 31, 33,
+                                     210, 311, 314, 318, 319, 321, 322, 324, 328, 330, 331, 333, 410])
 def test_mixed_steps_track_fp64(variant):
     """FP32 storage/stencils with the compensated update stay within the
     stated error of the FP64 oracle: 1e-5 relative L2 per field channel
