@@ -8,9 +8,16 @@ seeded thermal initial condition (Bose-Einstein modes at T = 0.5, built on
 the device with hipFFT); ``--ic modes`` uses a smooth synthetic field instead.
 
 Prints ONE JSON line (rank 0):
-  value      = site-updates/s (lattice sites x RK4 steps / second)
+  value      = site-updates/s (lattice sites x RK4 steps / second), FP64
   roofline   = achieved algorithmic HBM bandwidth of the dominant stage kernel
                against MI355X's 8 TB/s, from HIP-event launch timings
+  mixed      = the same for the mixed-precision path (FP32 storage/stencils,
+               compensated update, FP64 reductions)
+  precision  = mixed against FP64 after --precision-steps steps (default 1000)
+               from the same initial state: per-channel relative L2, energy
+               drift of each path, Gauss-norm ratio
+  batch      = 48 batched realisations (16 each at 64^3, 96^3, 128^3), mixed,
+               --batch-steps steps (default 300), autotuned, graph-captured
   cpu_baseline = the CPU oracle (literal C restatement, OpenMP) on a bounded
                sample of the same workload, on this host's cores
 """
@@ -32,7 +39,7 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {14: 7, 15: 7, 16: 4, 17: 4, 18: 6, 19: 6, 21: 7, 23: 7, 24: 7, 26: 7, 27: 7, 28: 7, 29: 7, 30: 7, 31: 7, 32: 7, 33: 7}
+STORED_MASK = {15: 7, 24: 7, 31: 7, 32: 7, 33: 7}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:
@@ -65,24 +72,34 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # Bytes per site that one stage launch of the automatic variants moves across
 # the L2/fabric boundary at 512^3, measured with rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE on this bench: FETCH_SIZE x 2 (what that counter reads for a
-# plain device copy of known size) + WRITE_SIZE x 1.
+# plain device copy of known size) + WRITE_SIZE x 1.  Each entry names the
+# profile summary that holds the counters and the commit of the library that
+# was measured; a kernel changed since then needs a new counter run.
+TRAFFIC_SOURCE = {
+    "r1": "profiles/stage_traffic_512.md (library at commit be83b44 / 1ea9c7a)",
+}
 MEASURED_TRAFFIC = {
-    ("fp64", 332, 1): 1110, ("fp64", 332, 2): 1504, ("fp64", 332, 3): 1500, ("fp64", 332, 4): 1882,
-    ("fp64", 329, 1): 1107, ("fp64", 329, 2): 1521, ("fp64", 329, 3): 1498, ("fp64", 329, 4): 2401,
-    ("mixed", 333, 1): 625, ("mixed", 333, 2): 848, ("mixed", 333, 3): 842, ("mixed", 333, 4): 1324,
-    ("mixed", 331, 1): 632, ("mixed", 331, 2): 850, ("mixed", 331, 3): 845, ("mixed", 331, 4): 1495,
-    ("mixed", 330, 1): 626, ("mixed", 330, 2): 851, ("mixed", 330, 3): 855, ("mixed", 330, 4): 1495,
-    ("mixed", 328, 1): 633, ("mixed", 328, 2): 854, ("mixed", 328, 3): 848, ("mixed", 328, 4): 1361,
-    ("fp64", 324, 1): 1154, ("fp64", 324, 2): 1387, ("fp64", 324, 3): 1386, ("fp64", 324, 4): 2407,
-    ("fp64", 321, 1): 1113, ("fp64", 321, 2): 1395, ("fp64", 321, 3): 1415, ("fp64", 321, 4): 2349,
-    ("fp64", 311, 1): 849, ("fp64", 311, 2): 1630, ("fp64", 311, 3): 2212, ("fp64", 311, 4): 3725,
-    ("mixed", 315, 1): 628, ("mixed", 315, 2): 696, ("mixed", 315, 3): 698, ("mixed", 315, 4): 1358,
+    ("fp64", 332, 1): (1110, "r1"), ("fp64", 332, 2): (1504, "r1"), ("fp64", 332, 3): (1500, "r1"),
+    ("fp64", 332, 4): (1882, "r1"),
+    ("fp64", 324, 1): (1154, "r1"), ("fp64", 324, 2): (1387, "r1"), ("fp64", 324, 3): (1386, "r1"),
+    ("fp64", 324, 4): (2407, "r1"),
+    ("fp64", 311, 1): (849, "r1"), ("fp64", 311, 2): (1630, "r1"), ("fp64", 311, 3): (2212, "r1"),
+    ("fp64", 311, 4): (3725, "r1"),
+    ("mixed", 333, 1): (625, "r1"), ("mixed", 333, 2): (848, "r1"), ("mixed", 333, 3): (842, "r1"),
+    ("mixed", 333, 4): (1324, "r1"),
+    ("mixed", 331, 1): (632, "r1"), ("mixed", 331, 2): (850, "r1"), ("mixed", 331, 3): (845, "r1"),
+    ("mixed", 331, 4): (1495, "r1"),
+    ("mixed", 315, 1): (628, "r1"), ("mixed", 315, 2): (696, "r1"), ("mixed", 315, 3): (698, "r1"),
+    ("mixed", 315, 4): (1358, "r1"),
 }
 
 
 def measured_traffic(precision: str, variant: int, stage: int, n: int):
-    b = MEASURED_TRAFFIC.get((precision, variant, stage)) if n == 512 else None
-    return None if b is None else b * n ** 3
+    """(bytes per launch, source) of the dominant kernel at 512^3, or (None, None)."""
+    e = MEASURED_TRAFFIC.get((precision, variant, stage)) if n == 512 else None
+    if e is None:
+        return None, None
+    return e[0] * n ** 3, TRAFFIC_SOURCE[e[1]]
 
 
 def cpu_baseline(n: int = 128, steps: int = 16):
@@ -132,8 +149,17 @@ def stream_copy(nbytes: int = 4 << 30, reps: int = 10) -> int:
     return 0
 
 
-def batch_bench(a) -> int:
-    """48 independent realisations stepped as one batch (mixed precision)."""
+BATCH_SHAPES = [64] * 16 + [96] * 16 + [128] * 16
+
+
+def batch_leg(steps: int, warmup: int, check_solo: bool = True) -> dict:
+    """48 independent realisations (16 each at 64^3, 96^3, 128^3; even
+    members thermal with T uniform in [0.3, 0.7] and J_ext = mH^2, odd
+    members 6-20 bubbles; distinct seeds) stepped as one batch in mixed
+    precision: one launch per stage over every lattice, graph-captured step
+    pairs, tile shape from the autotuner.  With check_solo, one member of
+    each size is stepped alone with the same variant from the same initial
+    state and must come out bitwise equal."""
     import numpy as np
 
     import ewpt
@@ -142,39 +168,69 @@ def batch_bench(a) -> int:
     rng = np.random.default_rng(2024)
     dx = 0.25
     par = ewpt.Parameters(N=64, dx=dx, gy=0.35)
-    shapes = [64] * 16 + [96] * 16 + [128] * 16
-    lats, ps = [], []
-    print(f"bench: batch of {len(shapes)} lattices, warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    lats, ps, inits = [], [], []
+    print(f"bench: batch of {len(BATCH_SHAPES)} lattices, warmup={warmup} steps={steps}", file=sys.stderr, flush=True)
+
+    def init(lat, k):
+        kind, arg, seed = inits[k]
+        if kind == "thermal":
+            lat.init_thermal(arg, par.meff_sq, seed, "config")
+        else:
+            lat.init_bubbles(ic.bubble_table(lat.nx, arg, seed, "config"), "config", par.mH)
+
     try:
-        for k, n in enumerate(shapes):
+        for k, n in enumerate(BATCH_SHAPES):
             lat = ewpt.Lattice(n, dx, precision=_lib.EWPT_MIXED)
             lats.append(lat)
             if k % 2 == 0:
-                lat.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 5000 + k, "config")
+                inits.append(("thermal", float(rng.uniform(0.3, 0.7)), 5000 + k))
                 ps.append(par.step_params(jex=par.mH ** 2))   # thermal runs start with J_ext = mH^2
             else:
-                nb = int(rng.integers(6, 21))
-                lat.init_bubbles(ic.bubble_table(n, nb, 7000 + k, "config"), "config", par.mH)
+                inits.append(("bubbles", int(rng.integers(6, 21)), 7000 + k))
                 ps.append(par.step_params())
-        sites = sum(n ** 3 for n in shapes)
+            init(lat, k)
+        sites = sum(n ** 3 for n in BATCH_SHAPES)
         with ewpt.Batch(lats) as bt:
             variant, tune_ms = bt.tune(ps, 3)
-            if a.warmup:
-                bt.step(ps, a.warmup)
-            ms = bt.step_timed(ps, a.steps)
-            e = [lat.energies(p) for lat, p in zip(lats, ps)]
+            for k, lat in enumerate(lats):   # restart from the initial states (tuning steps use dt = 0)
+                init(lat, k)
+            if warmup:
+                bt.step(ps, warmup)
+            ms = bt.step_timed(ps, steps)
+        e = [lat.energies(p) for lat, p in zip(lats, ps)]
+        solo_ok = None
+        if check_solo:
+            solo_ok = True
+            for k in (0, 17, 33):            # one member per size, both IC kinds
+                with ewpt.Lattice(BATCH_SHAPES[k], dx, precision=_lib.EWPT_MIXED) as one:
+                    one.variant = variant % 1000
+                    init(one, k)
+                    one.step(ps[k], warmup + steps)
+                    solo_ok &= bool(np.array_equal(one.get_state(), lats[k].get_state(), equal_nan=True))
     finally:
         for lat in lats:
             lat.close()
-    value = sites * a.steps / (ms * 1e-3)
+    return {
+        "value": sites * steps / (ms * 1e-3), "unit": "site-updates/s", "ms_per_step": ms / steps,
+        "steps": steps, "warmup": warmup, "lattices": "16x64^3+16x96^3+16x128^3", "precision": "mixed",
+        "ic": "even members thermal (T in [0.3, 0.7], J_ext = mH^2), odd members 6-20 bubbles",
+        "variant": variant, "tuned_ms_per_step": tune_ms,
+        "all_finite": bool(all(np.all(np.isfinite(x[:7])) for x in e)),
+        "members_bitwise_equal_solo": solo_ok,
+    }
+
+
+def batch_bench(a) -> int:
+    """The batched configuration as its own line."""
+    r = batch_leg(a.steps, a.warmup)
     out = {
         "metric": "site-updates/s, 48 batched realisations (16 x 64^3, 16 x 96^3, 16 x 128^3), mixed precision",
-        "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": r["value"], "unit": "site-updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic: seeded thermal (T in [0.3, 0.7]) and bubble (6-20) ICs on the device",
         "config": {"workload": "batched RK4 steps, one launch per stage over all 48 lattices, hipGraph-captured",
-                   "lattices": "16x64^3+16x96^3+16x128^3", "variant": variant, "tuned_ms_per_step": tune_ms},
-        "all_finite": bool(all(np.all(np.isfinite(x[:7])) for x in e)),
+                   "lattices": r["lattices"], "variant": r["variant"], "tuned_ms_per_step": r["tuned_ms_per_step"]},
+        "all_finite": r["all_finite"], "members_bitwise_equal_solo": r["members_bitwise_equal_solo"],
     }
     print(json.dumps(out))
     return 0
@@ -348,63 +404,6 @@ def slab_bench(a) -> int:
     return 0
 
 
-def precision_study(a) -> int:
-    """Mixed path against FP64 from the same thermal IC over a long run: per
-    field-channel relative L2 error, energy drift of each, Gauss-norm ratio.
-    The two states cannot both sit in one GPU's memory at 512^3 (290 GB), so
-    the FP64 run is done first and its final state kept on the host."""
-    import numpy as np
-
-    import ewpt
-    from ewpt import _lib
-
-    n, steps, chunk = a.n, a.study_steps, 100
-    par = ewpt.Parameters(N=n, dx=0.25, gy=0.35, T=0.5)
-    p = par.step_params()
-    out = {}
-    finals = {}
-    for prec in ("fp64", "mixed"):
-        lat = ewpt.Lattice(n, 0.25, precision=_lib.EWPT_FP64 if prec == "fp64" else _lib.EWPT_MIXED)
-        try:
-            lat.init_thermal(0.5, par.meff_sq, par.base_seed, "config")
-            e0, g0 = lat.energies(p), lat.gauss(p)
-            t0 = time.perf_counter()
-            done = 0
-            while done < steps:
-                k = min(chunk, steps - done)
-                lat.step(p, k)
-                done += k
-                print(f"bench: precision study {prec} {done}/{steps} steps ({time.perf_counter() - t0:.0f} s)",
-                      file=sys.stderr, flush=True)
-            wall = time.perf_counter() - t0
-            e1, g1 = lat.energies(p), lat.gauss(p)
-            finals[prec] = lat.get_state()
-        finally:
-            lat.close()
-        out[prec] = {"wall_s": wall, "site_updates_per_s": n ** 3 * steps / wall,
-                     "energy_drift": float((sum(e1[:7]) - sum(e0[:7])) / sum(e0[:7])),
-                     "gauss_l2_start": float(np.sqrt(np.sum(np.asarray(g0[:4]) ** 2))),
-                     "gauss_l2_end": float(np.sqrt(np.sum(np.asarray(g1[:4]) ** 2))),
-                     "aux_l2_end": float(np.sqrt(np.sum(np.asarray(g1[4:]) ** 2)))}
-    ref, mix = finals["fp64"], finals["mixed"]
-    rel = []
-    for c in range(36):
-        den = float(np.linalg.norm(ref[c]))
-        rel.append(float(np.linalg.norm(mix[c] - ref[c]) / den) if den > 0 else float(np.linalg.norm(mix[c])))
-    del finals, ref, mix
-    res = {
-        "metric": f"mixed vs FP64 after {steps} RK4 steps at {n}^3 (thermal IC, T=0.5)",
-        "value": max(rel[:16]), "unit": "max relative L2 error over the 16 field channels",
-        "higher_is_better": False, "n_gpus": 1, "steps": steps,
-        "field_rel_l2": rel[:16], "aux_rel_l2": rel[16:20], "momentum_rel_l2": rel[20:],
-        "speedup_mixed_vs_fp64": out["mixed"]["site_updates_per_s"] / out["fp64"]["site_updates_per_s"],
-        "gauss_ratio_mixed_over_fp64": out["mixed"]["gauss_l2_end"] / out["fp64"]["gauss_l2_end"],
-        "runs": out,
-    }
-    print(json.dumps(res))
-    return 0
-
-
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,6 +416,11 @@ def main(argv=None) -> int:
     ap.add_argument("--ic", choices=("thermal", "modes"), default="thermal",
                     help="initial state: thermal IC (T=0.5) or a smooth seeded mode sum")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--precision-steps", type=int, default=1000,
+                    help="steps of the FP64-vs-mixed comparison from the same initial state (0 = off); the "
+                         "warmup and timed steps are the first ones")
+    ap.add_argument("--batch-steps", type=int, default=300,
+                    help="steps of the batched 48-realisation leg in the default line (0 = off)")
     ap.add_argument("--batch", action="store_true",
                     help="the batched configuration instead: 48 realisations (16 each at 64^3, 96^3, 128^3), "
                          "half thermal, half bubbles, mixed precision, autotuned, graph-captured steps")
@@ -429,9 +433,6 @@ def main(argv=None) -> int:
     ap.add_argument("--corr-iters", type=int, default=3000)
     ap.add_argument("--corr-lr", type=float, default=3e-4)
     ap.add_argument("--corr-tol", type=float, default=1e-4)
-    ap.add_argument("--precision-study", action="store_true",
-                    help="mixed vs FP64 over --study-steps steps from the same thermal IC (errors, drifts)")
-    ap.add_argument("--study-steps", type=int, default=1000)
     ap.add_argument("--slab", action="store_true",
                     help="one GPU's share of the --n lattice split into --shards z-slabs, with halo exchange")
     ap.add_argument("--shards", type=int, default=8)
@@ -441,9 +442,6 @@ def main(argv=None) -> int:
 
     if a.stream_copy:
         return stream_copy()
-
-    if a.precision_study:
-        return precision_study(a)
 
     if a.slab:
         if a.gpus != 1:
@@ -469,6 +467,8 @@ def main(argv=None) -> int:
               file=sys.stderr)
         return 2
 
+    import numpy as np
+
     import ewpt
     from ewpt import _lib
 
@@ -478,8 +478,18 @@ def main(argv=None) -> int:
     params = ewpt.Parameters(N=n, dx=dx, gy=0.35, T=T)
     P = params.step_params()
     sites = n ** 3
+    total_steps = max(a.precision_steps, a.warmup + a.steps)
+    try:
+        import torch
+        dev_ok = torch.cuda.is_available()
+    except ImportError:
+        dev_ok = False
+    finals = {}
 
     def run(precision: str, variant: int) -> dict:
+        """IC, warmup, the timed steps, then (precision leg) the remaining
+        steps up to total_steps from the same IC with the same scalars; the
+        final state is kept on the device for the comparison."""
         prec = _lib.EWPT_FP64 if precision == "fp64" else _lib.EWPT_MIXED
         print(f"bench: {n}^3 {precision} variant={variant} warmup={a.warmup} steps={a.steps}",
               file=sys.stderr, flush=True)
@@ -490,11 +500,28 @@ def main(argv=None) -> int:
                 lat.init_thermal(T, params.meff_sq, params.base_seed, "config")
             else:
                 lat.init_modes(seed=2024, nmodes=4, kmax=3, amp=0.3)
+            e0, g0 = lat.energies(P), lat.gauss(P)
             if a.warmup:
                 lat.step(P, a.warmup)
             stage_ms, total_ms = lat.step_timed(P, a.steps)   # synchronises before returning
             active = lat.active_variant
-            e = lat.energies(P)
+            e_timed = lat.energies(P)
+            done = a.warmup + a.steps
+            if a.precision_steps:
+                t0 = time.perf_counter()
+                while done < total_steps:
+                    k = min(100, total_steps - done)
+                    lat.step(P, k)
+                    done += k
+                    print(f"bench: precision leg {precision} {done}/{total_steps} steps "
+                          f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+                if dev_ok:
+                    buf = torch.empty(lat.shape, dtype=torch.float64, device="cuda")
+                    lat.get_state_device(buf.data_ptr())
+                else:
+                    buf = lat.get_state()
+                finals[precision] = buf
+            e1, g1 = lat.energies(P), lat.gauss(P)
         finally:
             lat.close()
         value = sites * a.steps / (total_ms * 1e-3)
@@ -504,20 +531,29 @@ def main(argv=None) -> int:
         own = stage_bytes_per_site(precision, active)
         dom = max(range(4), key=lambda s: stage_ms[s]) + 1
         achieved = canon[dom] * sites / (stage_ms[dom - 1] * 1e-3)
+        traffic, source = measured_traffic(precision, active, dom, n)
+        if traffic is None and n == 512:
+            print(f"bench: WARNING no measured traffic for {precision} variant {active} stage {dom}: "
+                  "roofline.traffic is null", file=sys.stderr, flush=True)
         return {
             "value": value,
             "ms_per_step": total_ms / a.steps,
             "variant": active,
             "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK, "traffic": measured_traffic(precision, active, dom, n),
+                         "frac": achieved / HBM_PEAK, "traffic": traffic,
                          "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE)",
+                         "traffic_source": source,
                          "kernel": f"stage {dom} ({precision}, variant {active})", "bytes_per_site": canon[dom],
                          "variant_bytes_per_site": own[dom],
                          "variant_frac": own[dom] * sites / (stage_ms[dom - 1] * 1e-3) / HBM_PEAK},
             # the step against the minimal (formed-input) traffic of an RK4 step
             "step_roofline_frac": value * sum(STAGE_BYTES[precision].values()) / HBM_PEAK,
             "stage_ms": stage_ms,
-            "energy_check": float(sum(e[:7])),
+            "energy_check": float(sum(e_timed[:7])),
+            "steps_run": done,
+            "energy_drift": float((sum(e1[:7]) - sum(e0[:7])) / sum(e0[:7])),
+            "gauss_l2_start": float(np.sqrt(np.sum(np.asarray(g0[:4]) ** 2))),
+            "gauss_l2_end": float(np.sqrt(np.sum(np.asarray(g1[:4]) ** 2))),
         }
 
     # same-run copy ceiling: libewpt's one-double-per-lane copy of 4 GiB
@@ -559,6 +595,32 @@ def main(argv=None) -> int:
                         "roofline": m["roofline"], "step_roofline_frac": m["step_roofline_frac"],
                         "stage_ms": m["stage_ms"], "energy_check": m["energy_check"],
                         "energy_rel_diff_vs_fp64": abs(m["energy_check"] - head["energy_check"]) / abs(head["energy_check"])}
+    if len(finals) == 2:
+        ref, mix = finals["fp64"], finals["mixed"]
+        rel = []
+        for c in range(36):
+            if dev_ok:
+                den = float(torch.linalg.vector_norm(ref[c]))
+                num = float(torch.linalg.vector_norm(mix[c] - ref[c]))
+            else:
+                den = float(np.linalg.norm(ref[c]))
+                num = float(np.linalg.norm(mix[c] - ref[c]))
+            rel.append(num / den if den > 0 else num)
+        del finals["fp64"], finals["mixed"], ref, mix
+        f, m = res["fp64"], res["mixed"]
+        out["precision"] = {
+            "workload": f"mixed vs FP64 from the same {a.ic} initial state, {total_steps} RK4 steps at {n}^3, "
+                        "same scalars (the timed steps included)",
+            "steps": total_steps,
+            "max_field_rel_l2": max(rel[:16]),
+            "field_rel_l2": rel[:16], "aux_rel_l2": rel[16:20], "momentum_rel_l2": rel[20:],
+            "energy_drift_fp64": f["energy_drift"], "energy_drift_mixed": m["energy_drift"],
+            "energy_drift_diff": m["energy_drift"] - f["energy_drift"],
+            "gauss_l2_fp64": f["gauss_l2_end"], "gauss_l2_mixed": m["gauss_l2_end"],
+            "gauss_ratio_mixed_over_fp64": (m["gauss_l2_end"] / f["gauss_l2_end"]) if f["gauss_l2_end"] > 0 else None,
+        }
+    if a.batch_steps:
+        out["batch"] = batch_leg(a.batch_steps, 2)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

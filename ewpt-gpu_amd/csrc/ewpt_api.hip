@@ -10,6 +10,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -205,20 +206,19 @@ int push_view(ewpt_ctx *c)
 
 // variant actually launched: AUTO takes the first brick variant that tiles
 // the lattice from the measured ranking (512^3 on MI355X, FP64: 332 141.0,
-// 329 145.3, 327 148.7, 324 170.3, 321 177.3, 320 181.2, 311 184.4, 310 187.0 ms/step;
-// mixed: 333 78.9, 331 82.7, 330 83.4, 328 85.5, 315 89.4, 215 92.1, 311 96.3, 211
-// 103.2 ms/step), else the gather kernel.  The stored-stage-input variants (329,
-// 331, ...) hold one more state-sized buffer; without room for it (or on a
-// slab) AUTO skips them.
+// 324 170.3, 320 181.2, 311 184.4, 310 187.0 ms/step; mixed: 333 78.9,
+// 331 82.7, 315 89.4, 311 96.3, 211 103.2 ms/step), else the gather kernel.
+// The stored-stage-input variants (332, 333, ...) hold one more state-sized
+// buffer; without room for it (or on a slab) AUTO skips them.
 int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
     if (mixed(c)) {
-        for (int v : {333, 331, 330, 328, 315, 215, 311, 211, 310, 210})
+        for (int v : {333, 331, 315, 324, 311, 211, 310, 210, 325})
             if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported_f32(v, c->lat)) return v;
         return -1;
     }
-    for (int v : {332, 329, 327, 324, 321, 320, 311, 310, 210, 211, 2})
+    for (int v : {332, 324, 325, 320, 311, 310, 210, 211, 2})
         if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported(v, c->lat)) return v;
     return VARIANT_GATHER;
 }
@@ -376,7 +376,8 @@ int create_ctx(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devic
         return EWPT_ERR_INVAL;
     // stage kernels address a channel with 32-bit byte offsets
     if (zg && (long)lat->nx * lat->ny * (lat->nz + 2L * zg) * 8 >= (1L << 31)) return EWPT_ERR_UNSUPPORTED;
-    if (precision == EWPT_MIXED && !brick_supported_f32(210, *lat) && !brick_supported_f32(211, *lat))
+    if (precision == EWPT_MIXED && !brick_supported_f32(210, *lat) && !brick_supported_f32(211, *lat) &&
+        !brick_supported_f32(325, *lat))
         return EWPT_ERR_UNSUPPORTED;   // the mixed path has LDS-brick kernels only
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device >= ndev) {
@@ -487,6 +488,34 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host)
                               sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
            "get_state copy");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "get_state sync");
+    return EWPT_OK;
+}
+
+int ewpt_set_state_device(ewpt_ctx *ctx, const double *dev_state)
+{
+    if (!ctx || !dev_state) return EWPT_ERR_INVAL;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, copy_channels(ctx->y, ctx->n3, ctx->zg * ctx->plane, dev_state, sites(ctx), 0, NCH, sites(ctx),
+                              sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+           "set_state_device copy");
+    if ((rc = push_view(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "set_state_device sync");
+    ctx->has_state = true;
+    return EWPT_OK;
+}
+
+int ewpt_get_state_device(ewpt_ctx *ctx, double *dev_state)
+{
+    if (!ctx || !dev_state) return EWPT_ERR_INVAL;
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "get_state_device before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if ((rc = ensure_view(ctx))) return rc;
+    HIPCHK(ctx, copy_channels(dev_state, sites(ctx), 0, ctx->y, ctx->n3, ctx->zg * ctx->plane, NCH, sites(ctx),
+                              sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
+           "get_state_device copy");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "get_state_device sync");
     return EWPT_OK;
 }
 
@@ -1014,8 +1043,7 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
     if (ctx->zg && (variant == VARIANT_GATHER || brick_stored_inputs(variant)))
         return fail(ctx, EWPT_ERR_UNSUPPORTED, "slab contexts take the brick variants without stored stage inputs");
     if (mixed(ctx)) {
-        if (variant % 100 < 10 || variant % 100 > 33) return EWPT_ERR_INVAL;
-        if (!brick_exists(variant)) return EWPT_ERR_INVAL;
+        if (!brick_exists_f32(variant)) return EWPT_ERR_INVAL;
         if (!brick_supported_f32(variant, ctx->lat))
             return fail(ctx, EWPT_ERR_UNSUPPORTED, "stage variant does not tile this lattice");
         if (int rc = ensure_stage_input_buffer(ctx, variant)) return rc;
@@ -1068,6 +1096,7 @@ struct ewpt_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_beta = -1;    // beta flavour the captured graph was built for
+    bool no_stored = false; // automatic choice: a context had no room for a stage-input buffer
     std::string err;
 };
 
@@ -1101,8 +1130,8 @@ void drop_plan(ewpt_batch *b)
 // candidate variants in preference order (measured single-lattice ranking)
 std::vector<int> batch_candidates(bool f32)
 {
-    std::vector<int> v = f32 ? std::vector<int>{333, 331, 330, 328, 315, 215, 311, 211, 411, 310, 210, 15, 11, 10}
-                             : std::vector<int>{311, 310, 211, 210, 303, 203, 11, 10, 3, 2};
+    std::vector<int> v = f32 ? std::vector<int>{333, 331, 315, 311, 211, 310, 210, 11, 10}
+                             : std::vector<int>{332, 311, 310, 211, 210, 11, 10, 2};
     const size_t n = v.size();
     for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms
     return v;
@@ -1119,21 +1148,29 @@ int batch_resolved(const ewpt_batch *b)
 {
     if (b->variant) return b->variant;
     for (int v : batch_candidates(b->precision == EWPT_MIXED))
-        if (batch_fits_all(b, v)) return v;
+        if (batch_fits_all(b, v) && (!b->no_stored || !brick_stored_inputs(v % 1000))) return v;
     return -1;
 }
 
 int ensure_plan(ewpt_batch *b)
 {
     if (b->plan) return EWPT_OK;
-    const int v = batch_resolved(b);
+    int v = batch_resolved(b);
     if (v < 0) return bfail(b, EWPT_ERR_UNSUPPORTED, "no batch variant tiles every lattice");
     std::vector<ewpt_lattice> lats;
     for (ewpt_ctx *c : b->ctx) {
         lats.push_back(c->lat);
         // stored-stage-input variants need each context's stage-input buffer
-        if (int rc = ensure_stage_input_buffer(c, v % 1000))
-            return bfail(b, rc, ("batch: " + c->err).c_str());
+        int rc = ensure_stage_input_buffer(c, v % 1000);
+        if (rc == EWPT_ERR_OOM && !b->variant) {
+            // the automatic choice falls back to a candidate without that buffer
+            c->err.clear();
+            b->no_stored = true;
+            v = batch_resolved(b);
+            if (v < 0) return bfail(b, EWPT_ERR_OOM, "batch: no room for stage inputs and no variant without them");
+            rc = EWPT_OK;
+        }
+        if (rc) return bfail(b, rc, ("batch: " + c->err).c_str());
     }
     const char *what = "";
     b->plan = batch_plan_create(v, b->precision == EWPT_MIXED, lats, b->device, &what);
@@ -1281,10 +1318,30 @@ std::string batch_key(const ewpt_batch *b)
     return k;
 }
 
-std::map<std::string, int> &tune_cache()
+// tuned variant per set of lattice shapes, shared by every batch of the
+// process; guarded because batches may be tuned from several threads
+struct TuneCache {
+    std::mutex mu;
+    std::map<std::string, int> m;
+    bool get(const std::string &k, int &v)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = m.find(k);
+        if (it == m.end()) return false;
+        v = it->second;
+        return true;
+    }
+    void put(const std::string &k, int v)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        m[k] = v;
+    }
+};
+
+TuneCache &tune_cache()
 {
-    static std::map<std::string, int> m;
-    return m;
+    static TuneCache c;
+    return c;
 }
 
 }  // namespace
@@ -1618,11 +1675,11 @@ int ewpt_batch_tune(ewpt_batch *b, const ewpt_params *p, int nsteps, int *best_v
 {
     if (!b || !p || nsteps < 1) return EWPT_ERR_INVAL;
     const std::string key = batch_key(b);
-    auto hit = tune_cache().find(key);
-    if (hit != tune_cache().end()) {
-        int rc = ewpt_batch_set_variant(b, hit->second);
+    int cached = 0;
+    if (tune_cache().get(key, cached)) {
+        int rc = ewpt_batch_set_variant(b, cached);
         if (rc) return rc;
-        if (best_variant) *best_variant = hit->second;
+        if (best_variant) *best_variant = cached;
         if (best_ms) *best_ms = -1.0;   // served from the cache
         return EWPT_OK;
     }
@@ -1643,7 +1700,7 @@ int ewpt_batch_tune(ewpt_batch *b, const ewpt_params *p, int nsteps, int *best_v
         if (ms < tbest) { tbest = ms; best = v; }
     }
     if (best < 0) return bfail(b, EWPT_ERR_UNSUPPORTED, "no batch variant tiles every lattice");
-    tune_cache()[key] = best;
+    tune_cache().put(key, best);
     int rc = ewpt_batch_set_variant(b, best);
     if (rc) return rc;
     if (best_variant) *best_variant = best;

@@ -80,16 +80,15 @@ bool tile_of(int v, bool f32, int &tx, int &ty, int &tz)
     default: break;
     }
     if (f32) {
-        // stored stage inputs for stages 2-4 (the mixed path's best single-lattice form)
+        // stored stage inputs for stages 2-4 (the mixed path's single-lattice forms)
         switch (v) {
-        case 14: tx = 16; ty = 8; tz = 4; return true;
-        case 15: case 28: case 30: case 31: case 33: tx = 32; ty = 4; tz = 4; return true;
+        case 15: case 31: case 33: tx = 32; ty = 4; tz = 4; return true;
         default: return false;
         }
     }
     switch (v) {
     case 2: tx = 8; ty = 8; tz = 8; return true;
-    case 3: tx = 16; ty = 8; tz = 4; return true;
+    case 32: tx = 16; ty = 4; tz = 4; return true;
     default: return false;
     }
 }
@@ -197,16 +196,13 @@ auto with_kern(int v, F &&f)
 {
     if (v == 11) return f(Kern<32, 4, 4, 4432, T>{});
     if constexpr (std::is_same_v<T, float>) {
-        if (v == 14) return f(Kern<16, 8, 4, 704432, T>{});
         if (v == 15) return f(Kern<32, 4, 4, 704432, T>{});
-        if (v == 28) return f(Kern<32, 4, 4, 10704432, T>{});
-        if (v == 30) return f(Kern<32, 4, 4, 1010704432, T>{});
         if (v == 31) return f(Kern<32, 4, 4, 1010704443, T>{});
         if (v == 33) return f(Kern<32, 4, 4, 1210704443, T>{});
     }
     if constexpr (std::is_same_v<T, double>) {
         if (v == 2) return f(Kern<8, 8, 8, 1, T>{});
-        if (v == 3) return f(Kern<16, 8, 4, 2, T>{});
+        if (v == 32) return f(Kern<16, 4, 4, 211704432, T>{});
     }
     return f(Kern<16, 8, 4, 4432, T>{});
 }

@@ -28,6 +28,7 @@
 // first (n is parked in LDS until k4 is known).
 #pragma once
 
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -933,17 +934,24 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
     // stage 4 (never paired) parks n (FP64) or the increment and the high word (mixed)
     const size_t lds4 =
         2 * B::BUF * sizeof(T) + (std::is_same_v<T, float> && !lean_stash(PF) ? 2 : 1) * NK * B::NT * sizeof(T);
-    static bool attr_set = false;   // the kernels may need more than 64 KB of LDS
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4, T>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
-        const void *k13[3] = {reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 1, T>),
-                              reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 2, T>),
-                              reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 3, T>)};
-        for (int i = 0; i < 3 && e == hipSuccess; i++)
-            e = hipFuncSetAttribute(k13[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds13);
-        if (e != hipSuccess) return e;
-        attr_set = true;
+    // the kernels may need more than 64 KB of LDS: raise their limit once per
+    // process (a failed attempt is retried by the next launch); the mutex
+    // keeps threads that launch the same kernels from racing on the flag
+    static std::mutex attr_mu;
+    static bool attr_set = false;
+    {
+        std::lock_guard<std::mutex> g(attr_mu);
+        if (!attr_set) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 4, T>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds4);
+            const void *k13[3] = {reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 1, T>),
+                                  reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 2, T>),
+                                  reinterpret_cast<const void *>(&stage_brick<TX, TY, TZ, PF, BETA, 3, T>)};
+            for (int i = 0; i < 3 && e == hipSuccess; i++)
+                e = hipFuncSetAttribute(k13[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds13);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
     }
     switch (stage) {
     case 1: stage_brick<TX, TY, TZ, PF, BETA, 1, T><<<grid, block, lds13, st>>>(b, g, P, dt); break;

@@ -1,6 +1,8 @@
-// ewpt_brick_v32.hip -- stage-kernel variant 32 (FP64): as 29 with the field
-// channels of y' updated in the spread passes from the loads of the partial
-// sums (fused_fields), so stage 4 reads each momentum and k value once.
+// ewpt_brick_v32.hip -- stage-kernel variant 32 (FP64 default): variant 24 with
+// the next stage input's y loaded during the last passes of stages 2-3
+// (y_early), stage 4's partial-sum inputs loaded one channel per pass
+// (spread_update) and the field channels of y' updated from those loads
+// (fused_fields): each momentum and k value is read once.
 #include "ewpt_brick.h"
 
 namespace ewpt {

@@ -42,8 +42,7 @@ int shell_spectrum(const double *field, const ewpt_lattice &L, const int cut[3],
 // number of distinct integer |K|^2 of an n0*n1*n2 lattice (Kc_bin_nums)
 long distinct_k2_count(int n0, int n1, int n2);
 
-// ewpt_brick.hip -- LDS-tiled fused stage kernels, variants 1..5
-int brick_variant_count();
+// ewpt_brick.hip -- LDS-tiled fused stage kernels (FP64 variants)
 bool brick_exists(int variant);
 // variants in which some of stages 2-4 read their input as one array written
 // by the stage before (StageBufs::u / unext)
@@ -52,7 +51,8 @@ bool brick_supported(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
                               const Phys &P, double dt, hipStream_t st);
 
-// ewpt_brick_f32.hip -- mixed-precision stage kernels (variants 10, 11 + 100*order)
+// ewpt_brick_f32.hip -- mixed-precision stage kernels
+bool brick_exists_f32(int variant);
 bool brick_supported_f32(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float> &b, const ewpt_lattice &L,
                                   const Phys &P, double dt, hipStream_t st);

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <random>
 #include <vector>
@@ -409,12 +410,16 @@ template <int EPI>
 hipError_t conv_launch(const unsigned short *X, const unsigned short *Wk, const float *bias, const float *zprev,
                        int nx, int ny, int nz, float *Z, unsigned short *A, hipStream_t st)
 {
+    static std::mutex attr_mu;   // several threads may launch the first convolution at once
     static bool attr = false;
-    if (!attr) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<EPI>),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS);
-        if (e != hipSuccess) return e;
-        attr = true;
+    {
+        std::lock_guard<std::mutex> g(attr_mu);
+        if (!attr) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<EPI>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS);
+            if (e != hipSuccess) return e;
+            attr = true;
+        }
     }
     const int nb = (nx / BRX) * (ny / BRY) * (nz / BRZ);
     conv_kernel<EPI><<<nb, 256, CONV_LDS, st>>>(X, Wk, bias, zprev, nx, ny, nz, Z, A);

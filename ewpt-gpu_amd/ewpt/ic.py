@@ -73,11 +73,16 @@ def bubble_centres_ref(N: tuple[int, int, int], bub_diam: int, no_bubbles: int, 
     return locs
 
 
-def bubble_table(N, no_bubbles: int, seed: int, profile="ref", bub_diam: int | None = None) -> np.ndarray:
+def bubble_table(N, no_bubbles: int, seed: int, profile="ref", bub_diam: int | None = None,
+                 with_angles: bool = False):
     """(no_bubbles, 7) rows (ib, jb, kb, p1, p2, p3, p4) for init_bubbles.
 
     N is an int or (nx, ny, nz).  ``"ref"`` needs ``bub_diam``
-    (``Parameters.bub_diam``)."""
+    (``Parameters.bub_diam``).  with_angles=True returns (table, angles):
+    for ``"ref"`` the (no_bubbles, 3) uniform draws that gen_phi turns into
+    the orientations (the reference's Hoft_arr, main_rk4.jl:5677, written as
+    Hoft_angles at 6946); for ``"config"``, whose orientations are uniform
+    unit quaternions with no Hopf draws, None."""
     N = (N, N, N) if np.isscalar(N) else tuple(N)
     prof = profile_id(profile)
     if no_bubbles < 0:
@@ -95,4 +100,6 @@ def bubble_table(N, no_bubbles: int, seed: int, profile="ref", bub_diam: int | N
         locs = rng.uniform(0.0, 1.0, size=(no_bubbles, 3)) * np.array(N, dtype=np.float64)
         q = rng.standard_normal(size=(no_bubbles, 4))
         orient = q / np.linalg.norm(q, axis=1, keepdims=True)
-    return np.ascontiguousarray(np.hstack([locs, orient]), dtype=np.float64)
+        hoft = None
+    table = np.ascontiguousarray(np.hstack([locs, orient]), dtype=np.float64)
+    return (table, hoft) if with_angles else table

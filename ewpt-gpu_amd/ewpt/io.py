@@ -22,7 +22,9 @@ Files and datasets mirror the reference's outputs:
 * ``data.ewpt`` = ``data.h5`` (gpu-kernels-rk4/main_rk4.jl:6936-6957):
   scalars ``Nx gp2 dx dt nte gamma T meff_sq t_sat_jex nsnaps dsnaps``,
   for bubble runs ``no_bubbles``, ``bubble_locations`` (nb x 3) and
-  ``Hoft_angles`` (nb x 3), ``energies`` (nsnaps+1 x 13) and
+  ``Hoft_angles`` (nb x 3, the uniform draws of the reference profile; a
+  ``"config"``-profile run, whose orientations are unit quaternions, writes
+  ``bubble_orientations`` (nb x 4) instead), ``energies`` (nsnaps+1 x 13) and
   ``y_slices_snaps`` (nsnaps+1 x 7 x Nx x Nz); plus ``B_linear_fft``
   (2 x nbins x 2: first and last spectrum, columns |K| and power), a dataset
   the reference keeps in memory (5307, 6066-6069, 7069-7072).

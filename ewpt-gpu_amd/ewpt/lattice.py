@@ -124,6 +124,16 @@ class Lattice:
         self._check(self._L.ewpt_get_state(self._h, _dptr(out)), "ewpt_get_state")
         return out
 
+    def set_state_device(self, ptr: int):
+        """Copy the state in from a device buffer of 36*nx*ny*nz doubles
+        (e.g. a torch tensor's data_ptr()) on the context's device."""
+        self._check(self._L.ewpt_set_state_device(self._h, ctypes.c_void_p(ptr)), "ewpt_set_state_device")
+
+    def get_state_device(self, ptr: int):
+        """Copy the state (FP64; for the mixed path hi + lo of the FP32 pair)
+        out to a device buffer of 36*nx*ny*nz doubles."""
+        self._check(self._L.ewpt_get_state_device(self._h, ctypes.c_void_p(ptr)), "ewpt_get_state_device")
+
     # ---- the hot path
     def rhs(self, state: np.ndarray, p) -> np.ndarray:
         """One rk4_step! evaluation on host arrays (main_rk4.jl:52-661)."""

@@ -55,9 +55,10 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
         if thermal_init:
             lat.init_thermal(P.T, P.meff_sq, seed, prof)
         else:
-            table = bubble_table(N, P.no_bubbles, seed, prof, bub_diam=P.bub_diam)
+            table, hoft = bubble_table(N, P.no_bubbles, seed, prof, bub_diam=P.bub_diam, with_angles=True)
             lat.init_bubbles(table, prof, mH)
             res["bubble_table"] = table
+            res["hoft_angles"] = hoft
 
         p0 = P.step_params()
         energies = np.zeros((P.nsnaps + 1, 13))
@@ -101,7 +102,10 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
                 t = res["bubble_table"]
                 w.add("no_bubbles", P.no_bubbles)
                 w.add("bubble_locations", t[:, :3])
-                w.add("Hoft_angles", t[:, 3:])
+                if res["hoft_angles"] is not None:
+                    w.add("Hoft_angles", res["hoft_angles"])        # nb x 3 uniform draws (5677, 6946)
+                else:
+                    w.add("bubble_orientations", t[:, 3:])          # config profile: unit quaternions
             w.add("energies", res["energies"])
             w.add("y_slices_snaps", res["slices"])
             w.add("B_linear_fft", res["spectrum"])

@@ -67,6 +67,14 @@ int ewpt_destroy(ewpt_ctx *ctx);
 int ewpt_set_state(ewpt_ctx *ctx, const double *host_state);
 int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
 
+/* The same on a device buffer of 36*nx*ny*nz doubles on the context's
+ * device (a CuArray the caller owns, as the reference's state arrays are):
+ * device-to-device copies, synchronous.  Replaces the copyto! of
+ * CuArrays into and out of the evolved fields (main_rk4.jl:5640-5698,
+ * 6094-6103) for callers that keep their state on the GPU. */
+int ewpt_set_state_device(ewpt_ctx *ctx, const double *dev_state);
+int ewpt_get_state_device(ewpt_ctx *ctx, double *dev_state);
+
 /* Measurement aid, no counterpart in the reference: a plain FP64 device copy
  * of n doubles with one 8-byte access per lane (the stage kernels' access
  * width), launched reps + 1 times on stream; *ms = mean time of the last
@@ -252,41 +260,35 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 /* Kernel variant of the stage launches.  All variants compute the same
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
- *   0 = automatic: the first of 332, 329, 327, 324, 321, 320, 311, 310,
- *       210, 211, 2 that tiles the lattice, else 9 (mixed: 333, 331, 330, 328,
- *       315, 215, 311, 211, 310, 210); the stored-stage-input variants (332, 329,
- *       327, 333, 331, 330, 328, 324, 321, 315, 215) only when their buffer fits in device
- *       memory and not on slab contexts
- *   1..8, 10..19 = LDS-brick stage kernel (brick, prefetch, block order):
- *          1 = 16x8x4 pf1, 2 = 8x8x8 pf1, 3 = 16x8x4 pf2, 4 = 8x8x4 pf1,
- *          5 = 16x8x4 pf1 linear order, 6 = 16x4x4 pf1, 7 = 8x8x4 pf2,
- *          8 = 16x4x4 pf2, 10 = 16x8x4 and 11 = 32x4x4 with prefetch
- *          depth 4/4/3/2 for stages 1..4; 12, 13 = 10, 11 with two channels
- *          per LDS barrier; 14, 15 = 10, 11 with stored stage inputs
- *          (stages 1-3 also write the next stage's input, 36 channels,
- *          which stages 2-4 read as one array); 16, 17 = the same for
- *          stage 4 only; 18, 19 = for stages 3 and 4; 20, 22 = 11, 10 with
- *          the E and Gamma/Sigma centre values loaded with their own
- *          passes; 21, 23 = 20, 22 with stored stage inputs for stages 2-4;
- *          24, 25 = 16x4x4 bricks of 256 threads (two workgroups per CU in
- *          FP64) as 21, 20; 26 = 8x8x4 bricks of 256 threads as 21;
- *          27 = 24 with stages 2-3 loading the y values of the next stage
- *          input during their last passes; 28 (mixed) = 15 with the same;
- *          29 (FP64) = 27 with stage 4 loading the update's partial-sum
- *          inputs one channel per pass; 30 (mixed) = 28 with stage 4 keeping
- *          only the increment in LDS (two workgroups per CU); 31 (mixed) =
- *          30 with prefetch depth 4/4/4/3; 32 (FP64) = 29 with the field
- *          channels of y' updated in the spread passes from the same loads;
- *          33 (mixed) = 31 with the same spread and fused stage 4.
- *          14..19, 21, 23, 24, 26..33 hold one more state-sized buffer
- *          (allocated when first selected).  All but 5 use the XCD-aware
- *          block order.  The mixed path takes 10..28, 30, 31 and 33.
- *   9 = gather kernel (one site per thread, any lattice size)
+ *   0 = automatic: FP64 the first of 332, 324, 325, 320, 311, 310, 210, 211,
+ *       2 that tiles the lattice, else 9; mixed the first of 333, 331, 315,
+ *       324, 311, 211, 310, 210, 325.  The stored-stage-input variants (15,
+ *       24, 31, 32, 33) only when their buffer fits in device memory and not
+ *       on slab contexts.
+ *   LDS-brick stage kernels (brick shape, threads, form):
+ *       2 (FP64) = 8x8x8, 512 threads, prefetch depth 1;
+ *      10 = 16x8x4, 512 threads, every stage input formed on load;
+ *      11 = 32x4x4, 512 threads, formed on load;
+ *      15 (mixed) = 11 with stored stage inputs (stages 1-3 also write the
+ *           next stage's input, 36 channels, which stages 2-4 read as one array);
+ *      20 (FP64) = 11 with the E and Gamma/Sigma centre values loaded with
+ *           their own passes (late centres);
+ *      24 = 16x4x4 bricks of 256 threads (two workgroups per CU in FP64),
+ *           late centres, stored stage inputs;
+ *      25 = 24 with every stage input formed on load;
+ *      31 (mixed) = 15 with the next input's y loaded during the last passes,
+ *           a lean stage-4 stash (two workgroups per CU), prefetch 4/4/4/3;
+ *      32 (FP64) = 24 with the next input's y loaded during the last passes
+ *           and stage 4's update loads spread over its passes (bitwise 24);
+ *      33 (mixed) = 31 with the same spread stage 4 (bitwise 31).
+ *       15, 24, 31, 32, 33 hold one more state-sized buffer (allocated when
+ *       first selected).  Prefetch depth 4/4/3/2 by stage unless stated.
+ *   9 = gather kernel (FP64; one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
- * bricks; 4 = cubes of 4x4x4 bricks), e.g. 111.
- * Setting a brick variant that does not tile the lattice returns
- * EWPT_ERR_UNSUPPORTED. */
+ * bricks; 4 = cubes of 4x4x4 bricks), e.g. 311.
+ * Setting a variant the precision does not have returns EWPT_ERR_INVAL; one
+ * that does not tile the lattice returns EWPT_ERR_UNSUPPORTED. */
 int ewpt_set_variant(ewpt_ctx *ctx, int variant);
 int ewpt_get_variant(ewpt_ctx *ctx, int *variant);
 /* The variant the next step launches (what 0 = automatic resolves to). */
@@ -316,10 +318,12 @@ int ewpt_batch_create(ewpt_batch **out, ewpt_ctx *const *ctxs, int n);
 int ewpt_batch_destroy(ewpt_batch *b);
 const char *ewpt_batch_last_error(ewpt_batch *b);
 /* Brick variant for every lattice (as ewpt_set_variant: 10/11 (+100*order)
- * for both precisions, also 2/3 for FP64 and the stored-stage-input forms
- * 14/15/28/30/31/33 for mixed, which allocate each context's stage-input buffer), plus
- * 1000 for the persistent launch form (a resident grid striding through the
- * bricks instead of one workgroup per brick); 0 = first that tiles all. */
+ * for both precisions, also 2 and the stored-stage-input form 32 for FP64 and
+ * the stored-stage-input forms 15/31/33 for mixed, which allocate each
+ * context's stage-input buffer), plus 1000 for the persistent launch form (a
+ * resident grid striding through the bricks instead of one workgroup per
+ * brick); 0 = the first candidate that tiles all (when a context has no room
+ * for a stage-input buffer, the first that needs none). */
 int ewpt_batch_set_variant(ewpt_batch *b, int variant);
 int ewpt_batch_get_variant(ewpt_batch *b, int *variant);
 /* nsteps RK4 steps of every lattice; p holds one ewpt_params per lattice. */

@@ -17,7 +17,7 @@ def test_canonical_fp64_and_mixed():
     assert bench.STAGE_BYTES["mixed"] == {1: 224, 2: 304, 3: 368, 4: 816}
     assert sum(bench.STAGE_BYTES["mixed"].values()) == 1712
     # every formed-input variant moves the canonical bytes
-    for v in (0, 2, 10, 11, 311, 320, 322):
+    for v in (0, 2, 10, 11, 311, 320, 325):
         assert bench.stage_bytes_per_site("fp64", v) == bench.STAGE_BYTES["fp64"]
 
 
@@ -25,17 +25,25 @@ def test_stored_stage_inputs():
     # stages 2-4 stored: S1 y + k1 + u2; S2 u2 + y + k2 + u3; S3 the same; S4 u4 + y + k1..k3 + y'
     full = {1: (36 + 20 + 36) * 8, 2: (36 + 36 + 20 + 36) * 8, 3: (36 + 36 + 20 + 36) * 8,
             4: (36 + 36 + 60 + 36) * 8}
-    for v in (14, 315, 321, 324):
+    for v in (315, 324, 332):
         assert bench.stage_bytes_per_site("fp64", v) == full
-    # stage 4 only: S3 forms its input (y, 16 momentum k1, k2), writes k3 and u4
-    s4 = bench.stage_bytes_per_site("fp64", 316)
-    assert s4[3] == (36 + 16 + 20 + 20 + 36) * 8 and s4[4] == full[4] and s4[1] == 448 and s4[2] == 608
-    mixed = bench.stage_bytes_per_site("mixed", 315)
+    assert sum(full.values()) == 4128
+    mixed = bench.stage_bytes_per_site("mixed", 333)
+    assert mixed[1] == (36 + 20 + 36) * 4
     assert mixed[4] == (36 + 36 + 60 + 36 + 36 + 36) * 4
 
 
-def test_measured_traffic_only_at_512():
-    assert bench.measured_traffic("fp64", 324, 4, 256) is None
-    assert bench.measured_traffic("fp64", 999, 4, 512) is None
-    t = bench.measured_traffic("fp64", 324, 4, 512)
-    assert t == bench.MEASURED_TRAFFIC[("fp64", 324, 4)] * 512 ** 3
+def test_measured_traffic_only_at_512_with_source():
+    assert bench.measured_traffic("fp64", 332, 4, 256) == (None, None)
+    assert bench.measured_traffic("fp64", 999, 4, 512) == (None, None)
+    t, src = bench.measured_traffic("fp64", 332, 4, 512)
+    assert t == bench.MEASURED_TRAFFIC[("fp64", 332, 4)][0] * 512 ** 3
+    assert src.startswith("profiles/") and "commit" in src
+
+
+def test_default_variants_have_measured_traffic():
+    """The automatic variants at 512^3 (FP64 332, mixed 333) carry counter
+    figures for all four stages, so the default line's traffic is never null."""
+    for prec, v in (("fp64", 332), ("mixed", 333)):
+        for s in (1, 2, 3, 4):
+            assert (prec, v, s) in bench.MEASURED_TRAFFIC

@@ -35,8 +35,8 @@ def _params(k):
 
 
 @pytest.mark.parametrize("precision,variant", [(ewpt.EWPT_MIXED, 210), (ewpt.EWPT_MIXED, 1010),
-                                               (ewpt.EWPT_MIXED, 214), (ewpt.EWPT_MIXED, 1014),
-                                               (ewpt.EWPT_FP64, 210), (ewpt.EWPT_FP64, 1003)])
+                                               (ewpt.EWPT_FP64, 210), (ewpt.EWPT_FP64, 1002),
+                                               (ewpt.EWPT_FP64, 332), (ewpt.EWPT_FP64, 1332)])
 def test_batch_bitwise_equals_single(precision, variant):
     shapes = [(32, 32, 32), (48, 48, 48), (64, 32, 16), (32, 48, 64)]
     print(f"batch precision={precision} variant={variant}", flush=True)
@@ -66,7 +66,7 @@ def test_batch_bitwise_equals_single(precision, variant):
             x.close()
 
 
-@pytest.mark.parametrize("variant", [333, 331, 330, 1328])
+@pytest.mark.parametrize("variant", [333, 331, 315, 1333])
 def test_batch_32_wide_mixed_forms_bitwise(variant):
     # the 32x4x4 stored-input forms (early y, lean stage-4 stash) on lattices they tile
     shapes = [(32, 32, 32), (64, 32, 16), (32, 64, 32)]
@@ -101,7 +101,7 @@ def test_batch_tune_leaves_states_and_caches():
             bt.step(fields.params(dt=0.0), 1)
             before = [a.get_state() for a in lats]
             v, ms = bt.tune(_params(0), 2)
-            assert v % 100 in (10, 11, 14, 15, 28, 30, 31, 33) and ms > 0
+            assert v % 100 in (10, 11, 15, 31, 33) and ms > 0
             assert bt.variant == v
             for a, s in zip(lats, before):
                 assert np.array_equal(a.get_state(), s)      # dt = 0 timing steps change nothing

@@ -95,24 +95,28 @@ def test_thermal_constraints_256():
     assert np.all(np.isfinite(e)) and e[2] > 0 and e[0] > 0
 
 
-def test_reference_bubble_run_64_200_steps():
+@pytest.mark.parametrize("profile", ["ref", "config"])
+def test_reference_bubble_run_64_200_steps(profile):
     """The CPU-runnable reference configuration: 64^3, dx = 0.25, dt = dx/10,
-    10 seeded bubbles (reference profile), gauge fields and momenta zero,
-    200 RK4 steps.  The device run from the device-built IC matches the C
-    oracle's run from the same state to 1e-11 relative L2 per field, with
-    matching energies and Gauss residuals."""
+    10 seeded bubbles, gauge fields and momenta zero, 200 RK4 steps; with the
+    reference's profile (sub-lattice centres, its sech-like profile) and with
+    the configuration's literal one (uniform centres, uniform SU(2)
+    orientations, (1 - tanh)/2 wall of radius 3/mH).  The device run from the
+    device-built IC matches the C oracle's run from the same state to 1e-11
+    relative L2 per field, with matching energies and Gauss residuals."""
     n, dx = 64, 0.25
     P = ewpt.Parameters(N=n, dx=dx, gy=0.35)
-    t = ic.bubble_table(n, 10, P.base_seed, "ref", bub_diam=P.bub_diam)
+    t = ic.bubble_table(n, 10, P.base_seed, profile, bub_diam=P.bub_diam)
     p = fields.params(gy=0.35, dt=P.dt)
-    print("reference bubble run 64^3 x 200", flush=True)
+    print(f"bubble run 64^3 x 200, profile {profile}", flush=True)
     with ewpt.Lattice(n, dx) as lat:
-        lat.init_bubbles(t, "ref", P.mH)
+        lat.init_bubbles(t, profile, P.mH)
         s0 = lat.get_state()
         lat.step(p, 200)
         got = lat.get_state()
         e_dev, g_dev = lat.energies(p), lat.gauss(p)
-    np.testing.assert_allclose(s0, ic_np.bubbles_state((n, n, n), dx, t, 0, P.mH), atol=1e-13, rtol=0)
+    np.testing.assert_allclose(s0, ic_np.bubbles_state((n, n, n), dx, t, ic.profile_id(profile), P.mH),
+                               atol=1e-13, rtol=0)
     want = cref.rk4(s0, p, dx, 200)
     num = np.sqrt(((got - want) ** 2).sum(axis=(1, 2, 3)))
     den = np.sqrt((want ** 2).sum(axis=(1, 2, 3)))

@@ -112,9 +112,7 @@ def test_init_modes_matches_cpu_rebuild():
 
 
 @pytest.mark.parametrize("beta", [0.0, 0.2])
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23,
-                                     24, 25, 26, 27, 29, 111, 211, 310, 311, 410, 312, 314, 415, 316, 317, 418, 320,
-                                     321, 322, 323, 324, 325, 326, 327, 329, 32, 332])
+@pytest.mark.parametrize("variant", [2, 9, 10, 11, 20, 24, 25, 32, 111, 211, 310, 311, 320, 324, 325, 332, 432])
 def test_stage_variants_match_oracle(variant, beta):
     shape = (32, 32, 32)
     print(f"variant {variant} beta {beta} shape={shape}", flush=True)
@@ -130,7 +128,7 @@ def test_stage_variants_match_oracle(variant, beta):
     assert err.max() < 1e-12, (variant, err.argmax(), err.max())
 
 
-@pytest.mark.parametrize("variant", [310, 311, 410, 411])
+@pytest.mark.parametrize("variant", [310, 311, 410, 411, 332, 432])
 def test_cube_orders_match_oracle(variant):
     # 64x32x32 holds whole cubes of bricks for both brick shapes
     shape = (64, 32, 32)
@@ -146,18 +144,16 @@ def test_cube_orders_match_oracle(variant):
 
 
 @pytest.mark.parametrize("precision", [ewpt.EWPT_FP64, ewpt.EWPT_MIXED])
-@pytest.mark.parametrize("stored,partner", [(14, 10), (15, 11), (314, 310), (215, 211), (316, 310), (317, 311),
-                                            (318, 310), (319, 311), (321, 320), (223, 222), (324, 325),
-                                            (327, 325), (328, 311), (329, 327), (330, 328), (331, 330)])
+@pytest.mark.parametrize("stored,partner", [(324, 325), (15, 11), (315, 311), (332, 325), (333, 311)])
 def test_stored_stage_inputs_match_partner(stored, partner, precision):
     """Variants that pass stage inputs on as one array form them with the same
     expressions as their partner variant forms them on load; the compiler may
     fuse a product into an add in one and not the other, so the two agree to
     a few roundings of the arithmetic type (64 eps of the largest value)."""
-    if precision == ewpt.EWPT_FP64 and stored % 100 in (28, 30, 31):
+    if precision == ewpt.EWPT_FP64 and stored % 100 in (15, 33):
         pytest.skip("a mixed-precision form")
-    if precision == ewpt.EWPT_MIXED and stored % 100 == 29:
-        pytest.skip("variant 29 is an FP64 form")
+    if precision == ewpt.EWPT_MIXED and stored % 100 == 32:
+        pytest.skip("variant 32 is an FP64 form")
     shape = (64, 32, 32)
     p = fields.params(beta_w=0.1, gamma=0.2, jex=0.3)
     s = fields.smooth_state(shape, seed=23, amp=0.5)
@@ -174,8 +170,7 @@ def test_stored_stage_inputs_match_partner(stored, partner, precision):
     assert d <= 64 * eps * np.abs(out[1]).max(), d
 
 
-@pytest.mark.parametrize("precision,form,base", [(ewpt.EWPT_FP64, 327, 324), (ewpt.EWPT_FP64, 329, 327), (ewpt.EWPT_FP64, 332, 329),
-                                                 (ewpt.EWPT_MIXED, 330, 328), (ewpt.EWPT_MIXED, 331, 330),
+@pytest.mark.parametrize("precision,form,base", [(ewpt.EWPT_FP64, 332, 324), (ewpt.EWPT_MIXED, 331, 315),
                                                  (ewpt.EWPT_MIXED, 333, 331)])
 def test_latency_forms_bitwise(precision, form, base):
     """Forms that only move loads earlier or later (the next input's y during
@@ -212,8 +207,11 @@ def test_brick_nonuniform_lattice():
 def test_variant_rejected_when_not_tiling():
     with ewpt.Lattice((24, 24, 24), DX) as lat:
         with pytest.raises(ewpt.EwptError) as ei:
-            lat.variant = 1          # 24 % 16 != 0
+            lat.variant = 10         # 24 % 16 != 0
         assert ei.value.code == ewpt._lib.EWPT_ERR_UNSUPPORTED
+        with pytest.raises(ewpt.EwptError) as ei:
+            lat.variant = 33         # a mixed-precision form
+        assert ei.value.code == ewpt._lib.EWPT_ERR_INVAL
         lat.variant = 2              # 8x8x8 tiles 24^3
 
 
@@ -229,8 +227,7 @@ def test_mixed_state_roundtrip():
     assert np.abs(y - s).max() <= 1e-13 * np.abs(s).max()
 
 
-@pytest.mark.parametrize("variant", [0, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 27, 28, 30, 31, 33,
-                                     210, 311, 314, 318, 319, 321, 322, 324, 328, 330, 331, 333, 410])
+@pytest.mark.parametrize("variant", [0, 10, 11, 15, 24, 25, 31, 33, 210, 311, 315, 324, 325, 331, 333, 410])
 def test_mixed_steps_track_fp64(variant):
     """FP32 storage/stencils with the compensated update stay within the
     stated error of the FP64 oracle: 1e-5 relative L2 per field channel
@@ -250,6 +247,27 @@ def test_mixed_steps_track_fp64(variant):
     assert err[:20].max() < 1e-5, err
     er = cref.energies(r, p, DX)
     np.testing.assert_allclose(e[:7], er[:7], rtol=1e-5, atol=1e-9)
+
+
+@pytest.mark.parametrize("variant", [0, 331, 333, 324, 311])
+def test_mixed_all_terms_track_fp64(variant):
+    """The mixed kernels with every term of the right-hand side on: the
+    gamma damping, J_ext, and the beta_W / beta_Y terms (zero in the
+    benchmark configurations, used by run() for thermal starts and damped
+    runs).  50 steps of a smooth 32^3 state stay within 1e-5 relative L2 per
+    field channel of the FP64 oracle."""
+    shape = 32
+    p = fields.params(gamma=0.3, jex=0.5, beta_w=0.2, beta_y=0.1)
+    s = fields.smooth_state(shape, seed=25, amp=0.4)
+    with ewpt.Lattice(shape, DX, precision=ewpt.EWPT_MIXED) as lat:
+        lat.variant = variant
+        lat.set_state(s)
+        lat.step(p, 50)
+        y = lat.get_state()
+    r = cref.rk4(s, p, DX, 50)
+    err = rel_l2_per_channel(y, r)
+    print(f"mixed variant {variant}, all terms: max field rel-L2 after 50 steps {err[:20].max():.3e}", flush=True)
+    assert err[:20].max() < 1e-5, err
 
 
 def test_mixed_rejects_gather_variant():

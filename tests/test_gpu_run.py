@@ -87,6 +87,11 @@ def test_driver_matches_oracle_evolution(thermal, tmp_path):
     np.testing.assert_array_equal(d["energies"], e)
     assert d["y_slices_snaps"].shape == (nsnaps + 1, 7, N, N)
     assert ("bubble_locations" in d) == (not thermal)
+    if not thermal:
+        # Hoft_angles holds the nb x 3 uniform draws of gen_phi (main_rk4.jl:5677, 6946)
+        assert d["Hoft_angles"].shape == (3, 3)
+        np.testing.assert_array_equal(d["Hoft_angles"], res["hoft_angles"])
+        assert np.all((d["Hoft_angles"] >= 0) & (d["Hoft_angles"] < 1))
     assert d["B_linear_fft"].shape[0] == 2 and d["B_linear_fft"].shape[2] == 2
     f = io.read(str(tmp_path / f"3d-fft-data-{nte // 2}.ewpt"))
     B = cref.em_fields(y_mid, base | dict(jex=0.0, gamma=0.0), DX, P.theta_w)
@@ -101,3 +106,4 @@ def test_cli(tmp_path):
     assert runmod.main(args + ["--bubbles", "--out", str(tmp_path)]) == 0
     d = io.read(str(tmp_path / "data.ewpt"))
     assert d["energies"].shape == (3, 13) and int(d["no_bubbles"]) == 2
+    assert d["Hoft_angles"].shape == (2, 3)

@@ -123,6 +123,13 @@ def test_bubble_table_reference_profile():
         ic.bubble_table(64, 126, 1, "ref", bub_diam=12)            # only 5^3 sub-lattice sites
     with pytest.raises(ValueError):
         ic.bubble_table(64, 3, 1, "ref")
+    # the Hopf draws behind the orientations: gen_phi of each row with one of
+    # the two signs gives that row's orientation (uniforms in [0, 1))
+    t2, hoft = ic.bubble_table(64, 10, P.base_seed, "ref", bub_diam=12, with_angles=True)
+    assert np.array_equal(t2, t) and hoft.shape == (10, 3)
+    assert np.all((hoft >= 0) & (hoft < 1))
+    for row, h in zip(t, hoft):
+        assert min(np.abs(ic.gen_phi(h, sg) - row[3:]).max() for sg in (-1.0, 1.0)) < 1e-15
 
 
 def test_bubble_table_config_profile():
@@ -130,6 +137,7 @@ def test_bubble_table_config_profile():
     assert t.shape == (20, 7)
     assert np.all(t[:, :3] >= 0) and np.all(t[:, :3] < np.array([64, 48, 32]))
     np.testing.assert_allclose(np.linalg.norm(t[:, 3:], axis=1), 1.0, atol=1e-15)
+    assert ic.bubble_table((64, 48, 32), 20, 99, "config", with_angles=True)[1] is None
     with pytest.raises(ValueError):
         ic.bubble_table(64, 3, 1, "tanh")
 

@@ -233,6 +233,147 @@ def test_higgs_mass_and_dispersion(oracle_lib):
     assert np.abs(ratio[mask] - 1).max() < 1e-5
 
 
+# ------------------------------------------- damping and beta terms (closed forms)
+#
+# gamma, beta_W and beta_Y are zero in every benchmark configuration, and the
+# covariance check above cannot see them (gamma and the beta_W bracket are not
+# covariant; beta_Y is SU(2)-invariant).  Each is pinned here by a state on
+# which the oracle's RHS reduces to a closed form read off the reference, and
+# each closed form is shown to differ from a plausible transcription slip, so
+# that the slip would fail the first assertion.
+
+def _uniform_state(shape, vals):
+    """State with channel c equal to vals.get(c, 0) everywhere."""
+    st = np.zeros((36,) + tuple(reversed(shape)))
+    for c, v in vals.items():
+        st[c] = v
+    return st
+
+
+@pytest.mark.parametrize("restatement", ["c", "numpy"])
+def test_gamma_damping_closed_form(oracle_lib, restatement):
+    """Uniform phi and pi, W = Y = Gamma = Sigma = 0, gw = gy = lambda = J_ext = 0:
+    every kt_phi term but the damping vanishes, so kt_phi_a = -gamma phi_a (phi_a pi_a)
+    / |phi|^2 exactly, per component (main_rk4.jl:349, 367, 385, 403; quirk q3).
+    The covariant form -gamma phi_a (phi.pi)/|phi|^2, a pi index slipped to
+    the next component and a missing square in the denominator all differ."""
+    phi = np.array([0.3, -0.5, 0.7, 1.1])
+    pi = np.array([0.4, 0.9, -0.6, 0.2])
+    st = _uniform_state((8, 8, 8), {**{a: phi[a] for a in range(4)}, **{20 + a: pi[a] for a in range(4)}})
+    p = fields.params(gw=0.0, gy=0.0, lambda_=0.0, jex=0.0, gamma=0.37)
+    out = (cref.rhs if restatement == "c" else rhs_np.rhs)(st, p, DX)
+    r2 = np.sum(phi ** 2)
+    want = -0.37 * phi * (phi * pi) / r2
+    got = out[20:24].reshape(4, -1)
+    # 1e-13: the lattice Laplacian of a constant is zero only to rounding (x 1/dx^2)
+    assert np.abs(got - want[:, None]).max() < 1e-13
+    assert np.abs(out[24:36]).max() < 1e-13 and np.abs(out[16:20]).max() < 1e-13   # nothing else moves E, Gamma
+    slips = [-0.37 * phi * np.dot(phi, pi) / r2,                   # summed over components
+             -0.37 * phi * (phi * np.roll(pi, -1)) / r2,           # pi_b of the next component
+             -0.37 * phi * (phi * pi) / np.sqrt(r2)]               # |phi| for |phi|^2
+    for w in slips:
+        assert np.abs(w - want).max() > 1e-3
+
+
+def _Fw(W, gw):
+    """F^a_ij of uniform W^a_i (field_strengths.jl:35-311, derivative terms
+    zero): F^a_ij = gw (W^b_i W^c_j - W^b_j W^c_i), (a, b, c) cyclic.
+    Returns F[a][(i, j)] for (i, j) in xy, xz, yz."""
+    F = {}
+    for a in range(3):
+        b, c = (a + 1) % 3, (a + 2) % 3
+        F[a] = {(i, j): gw * (W[b][i] * W[c][j] - W[b][j] * W[c][i]) for (i, j) in ((0, 1), (0, 2), (1, 2))}
+    return F
+
+
+@pytest.mark.parametrize("restatement", ["c", "numpy"])
+def test_beta_w_bracket_reads_e1_in_every_row(oracle_lib, restatement):
+    """The beta_W term of kt_phi_a is beta_W * B * phi_a with
+    B = sum_a (E^a_x F^a_yz - E^1_y F^a_xz + E^1_z F^a_xy): the a = 2, 3 rows
+    read E^1_y, E^1_z where E^a_y, E^a_z are expected (main_rk4.jl:349-352,
+    quirk q1).  On uniform fields F^a_ij = gw eps W^b_i W^c_j, so
+    kt_phi(beta_W) - kt_phi(0) is known in closed form:
+      only E^2_y non-zero -> 0 (the "expected" form would give -E^2_y F^2_xz);
+      only E^1_y, E^1_z non-zero -> -E^1_y sum_a F^a_xz + E^1_z sum_a F^a_xy
+      (the expected form: -E^1_y F^1_xz + E^1_z F^1_xy)."""
+    gw = 0.65
+    rng = np.random.default_rng(5)
+    W = rng.uniform(-0.8, 0.8, size=(3, 3))            # W[a][i]
+    phi = np.array([0.2, -0.4, 0.3, 0.9])
+    F = _Fw(W, gw)
+    base = {**{a: phi[a] for a in range(4)}, **{4 + 3 * a + i: W[a][i] for a in range(3) for i in range(3)}}
+    f = cref.rhs if restatement == "c" else rhs_np.rhs
+
+    def dkt(evals, bw=0.3):
+        st = _uniform_state((8, 8, 8), {**base, **evals})
+        on = f(st, fields.params(gw=gw, beta_w=bw), DX)[20:24].reshape(4, -1)
+        off = f(st, fields.params(gw=gw, beta_w=0.0), DX)[20:24].reshape(4, -1)
+        return on - off
+
+    e2y = 0.7
+    d = dkt({24 + 3 * 1 + 1: e2y})                      # E^2_y
+    assert np.abs(d).max() < 1e-13
+    expected_form = -0.3 * e2y * F[1][(0, 2)] * phi
+    assert np.abs(expected_form).max() > 1e-3
+    e1y, e1z = 0.5, -0.8
+    d = dkt({24 + 1: e1y, 24 + 2: e1z})                 # E^1_y, E^1_z
+    Bq = -e1y * sum(F[a][(0, 2)] for a in range(3)) + e1z * sum(F[a][(0, 1)] for a in range(3))
+    want = 0.3 * Bq * phi
+    assert np.abs(d - want[:, None]).max() < 1e-13
+    B_expected = -e1y * F[0][(0, 2)] + e1z * F[0][(0, 1)]
+    assert np.abs(0.3 * B_expected * phi - want).max() > 1e-3
+
+
+def _wave(shape, m, axis, fn):
+    """fn(theta j) along axis (0 = x, 1 = y, 2 = z) on the (nz, ny, nx) grid, theta = 2 pi m / n."""
+    n = shape[axis]
+    th = 2 * np.pi * m / n
+    shp = [1, 1, 1]
+    shp[2 - axis] = n
+    return np.broadcast_to(fn(th * np.arange(n)).reshape(shp), tuple(reversed(shape))).copy(), th
+
+
+@pytest.mark.parametrize("restatement", ["c", "numpy"])
+def test_beta_y_terms_closed_form(oracle_lib, restatement):
+    """beta_Y terms of kt_Y_i (main_rk4.jl:553-581) on two U(1)-only states,
+    gw = gy = lambda = 0 (K is the d1 symbol of diff_scheme.jl:157-233, and
+    Y_yz = d_y Y_z - d_z Y_y, field_strengths.jl:308-311):
+    (a) Y_z = A sin(theta y), phi_4 = v, pi_4 = p uniform: only the first
+        term survives, kt_Y_x = -2 beta_Y (phi.pi) Y_yz = -2 beta_Y v p A K cos(theta y);
+    (b) phi_4 = v + b sin(theta x), E^Y_y = e uniform, Y = pi = 0: only the
+        second term survives, kt_Y_z = 2 beta_Y ((phi.d_x phi) E^Y_y - (phi.d_y phi) E^Y_x)
+        = 2 beta_Y e (v + b sin(theta x)) b K cos(theta x), kt_Y_x = kt_Y_y = 0.
+    Plausible slips give other values: the first term with its sign or the
+    index order of Y_jk flipped, or |phi|^2 for phi.pi; E^Y_j and E^Y_k
+    swapped in the second (kt_Y_z would then vanish here)."""
+    shape = (16, 16, 16)
+    f = cref.rhs if restatement == "c" else rhs_np.rhs
+    by = 0.4
+    p = fields.params(gw=0.0, gy=0.0, lambda_=0.0, beta_y=by)
+    # (a)
+    A, v, pv = 0.3, 1.0, 0.6
+    st = _uniform_state(shape, {3: v, 23: pv})
+    st[15], th = _wave(shape, 2, 1, np.sin)
+    st[15] *= A
+    out = f(st, p, DX)
+    cosy, _ = _wave(shape, 2, 1, np.cos)
+    want = -2 * by * (v * pv) * A * D1(th) * cosy
+    assert np.abs(out[33] - want).max() < 1e-12 * np.abs(want).max()
+    for slip in (-want, want * v / pv):             # sign / Y_kj order; |phi|^2 for phi.pi
+        assert np.abs(slip - want).max() > 0.1 * np.abs(want).max()
+    # (b)
+    b, e = 0.2, 0.5
+    st = _uniform_state(shape, {34: e})
+    sx, thx = _wave(shape, 1, 0, np.sin)
+    cosx, _ = _wave(shape, 1, 0, np.cos)
+    st[3] = v + b * sx
+    out = f(st, p, DX)
+    want = 2 * by * e * (v + b * sx) * b * D1(thx) * cosx
+    assert np.abs(out[35] - want).max() < 1e-12 * np.abs(want).max()
+    assert np.abs(out[33]).max() < 1e-14 and np.abs(out[34]).max() < 1e-14
+    assert np.abs(want).max() > 1e-2                 # the swapped form gives 0 in kt_Y_z
+
+
 # --------------------------------------------------------------- RK4 loop
 
 def test_rk4_loop_bitwise_against_numpy_loop(oracle_lib):
