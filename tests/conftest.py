@@ -19,3 +19,15 @@ def oracle_lib():
     from oracle import cref
     cref.build()
     return cref
+
+
+def pytest_sessionstart(session):
+    """torch ships its own HIP runtime beside the system one that libewpt
+    links.  Tests that park a 512^3 state in a torch device buffer need
+    torch's runtime to come up before the library's; initialise it first
+    (a no-op on a machine without a GPU)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
