@@ -910,6 +910,15 @@ int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13])
     return EWPT_OK;
 }
 
+int ewpt_energies_device(const ewpt_lattice *lat, const ewpt_params *p, const double *dev_state, double *dev_out,
+                         void *stream)
+{
+    if (!lattice_ok(lat) || !params_ok(p) || !dev_state || !dev_out) return EWPT_ERR_INVAL;
+    if (!energies_noalloc_fits(*lat)) return EWPT_ERR_UNSUPPORTED;   // the 16x4x4-brick reduction only
+    const hipError_t e = launch_energies_noalloc(dev_state, *lat, phys_of(*p), dev_out, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? EWPT_OK : EWPT_ERR_HIP;
+}
+
 namespace {
 
 // scratch for the diagnostics: the stored stage derivatives are free between

@@ -6,6 +6,8 @@
 // the RK4 bookkeeping of ewpt_stage.h: the stage input y + c*k is formed on
 // load and stage 4 writes y' directly.  It works for any lattice size >= 4.
 // It also provides the energy and Gauss-law reductions.
+#include <algorithm>
+#include <mutex>
 #include <utility>
 
 #include "ewpt_internal.h"
@@ -345,26 +347,29 @@ __device__ __forceinline__ void energy_stage_all(const double *y, long n3, const
     (energy_stage_channel<Tag, Cs>(y, n3, g, i0, j0, k0, t, tile, s), ...);
 }
 
-__global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3))) energy_brick(const double *y, long n3, Geo g, Phys P, double *part)
+// The ten reduction slots of brick b at thread t's site (energy_density's
+// per-site values, computed on the LDS-staged brick).
+template <int Tag>
+__device__ __forceinline__ void energy_brick_site(const double *y, long n3, const Geo &g, const Phys &P, long b, int t,
+                                                  double *tile, double (&v)[10])
 {
-    __shared__ double tile[EBOX];
-    const int t = threadIdx.x;
     const int nbx = g.nx / EBX, nby = g.ny / EBY;
-    const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
     const int i0 = bx * EBX, j0 = by * EBY, k0 = bz * EBZ;
     const int tx = t % EBX, ty = (t / EBX) % EBY, tz = t / (EBX * EBY);
     const long q = (i0 + tx) + (long)g.nx * ((j0 + ty) + (long)g.ny * (k0 + tz));
     SiteIn<double> s;
-    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, INFINITY, 0};
+#pragma unroll
+    for (int qq = 0; qq < 10; qq++) v[qq] = 0.0;
 #pragma unroll
     for (int c = 0; c < NCH; c++)
         if (c < 16 || c >= CH_PI) s.c[c] = y[(long)c * n3 + q];
     energy_centre_terms(s, P, v);   // the momenta are not needed after this
     // phi passes, then the gradient term (the phi gradients die), then W and Y
-    energy_stage_all<0>(y, n3, g, i0, j0, k0, t, tile, s, std::integer_sequence<int, 0, 1, 2, 3>{});
+    energy_stage_all<Tag>(y, n3, g, i0, j0, k0, t, tile, s, std::integer_sequence<int, 0, 1, 2, 3>{});
     v[2] = energy_gradient_term(s, P);
-    energy_stage_all<0>(y, n3, g, i0, j0, k0, t, tile, s,
-                        std::integer_sequence<int, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>{});
+    energy_stage_all<Tag>(y, n3, g, i0, j0, k0, t, tile, s,
+                          std::integer_sequence<int, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>{});
     energy_magnetic_terms(s, P, v);
     const double m = sqrt(s.c[0] * s.c[0] + s.c[1] * s.c[1] + s.c[2] * s.c[2] + s.c[3] * s.c[3]);
     v[7] = m;
@@ -373,7 +378,31 @@ __global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3))) e
 #pragma unroll
     for (int qq = 0; qq < 8; qq++) bad += !isfinite(v[qq]);
     v[9] = (double)bad;
+}
+
+__global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3))) energy_brick(const double *y, long n3, Geo g, Phys P, double *part)
+{
+    __shared__ double tile[EBOX];
+    double v[10];
+    energy_brick_site<0>(y, n3, g, P, blockIdx.x, (int)threadIdx.x, tile, v);
     block_reduce<10>(v, 1u << 8, part);
+}
+
+// The same densities with a bounded number of partials: block b sums the
+// bricks b, b + gridDim.x, ... in that order before its block reduction, so
+// the partials need gridDim.x slots whatever the lattice size.
+__global__ void __launch_bounds__(EBT) __attribute__((amdgpu_waves_per_eu(3)))
+energy_brick_strided(const double *y, long n3, Geo g, Phys P, long nbrick, double *part)
+{
+    __shared__ double tile[EBOX];
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, INFINITY, 0};
+    for (long b = blockIdx.x; b < nbrick; b += gridDim.x) {   // the same trip count for every thread of a block
+        double v[10];
+        energy_brick_site<3>(y, n3, g, P, b, (int)threadIdx.x, tile, v);
+#pragma unroll
+        for (int q = 0; q < 10; q++) acc[q] = q == 8 ? fmin(acc[q], v[q]) : acc[q] + v[q];
+    }
+    block_reduce<10>(acc, 1u << 8, part);
 }
 
 bool energy_brick_fits(const ewpt_lattice &L)
@@ -622,6 +651,16 @@ __global__ void __launch_bounds__(256) finish_reduce(const double *part, long nb
     if (threadIdx.x == 0) res[q] = sh[0];
 }
 
+// The 13-column energy table from the reduced slots (the host-side scaling
+// of ewpt_energies, done on the device): columns 0..6 times dx^3, 9 min|phi|,
+// 11 mean|phi|, the rest 0.
+__global__ void energy_table(const double *res, double vol, double nsites, double *out)
+{
+    const int q = threadIdx.x;
+    if (q >= 13) return;
+    out[q] = q < 7 ? res[q] * vol : q == 9 ? res[8] : q == 11 ? res[7] / nsites : 0.0;
+}
+
 Geo make_geo(const ewpt_lattice &L)
 {
     Geo g;
@@ -726,6 +765,61 @@ hipError_t launch_em_energies(const double *y, const ewpt_lattice &L, const Phys
         em_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
         finish_reduce<<<4, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
     }
+    return hipGetLastError();
+}
+
+// ---- the allocation-free energy table (ewpt_energies_device)
+//
+// Partial sums live in a file-scope device array instead of memory the
+// library allocates.  Its host-side name is not a device address: the
+// address is taken once per device with hipGetSymbolAddress (that device
+// current) and only that address is handed to kernels.  Every call on a
+// device shares the one array; calls that overlap on one device race.  Each
+// block writes all of its slots, so nothing needs clearing between calls.
+constexpr int NOALLOC_BLOCKS = 2048;                 // blocks of energy_brick_strided, one partial row each
+constexpr int NOALLOC_DOUBLES = NOALLOC_BLOCKS * NRED + NRED;   // partials, then the reduced slots
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[MAX_DEVICES] = {};
+
+}  // namespace
+
+hipError_t scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
+}
+
+bool energies_noalloc_fits(const ewpt_lattice &L) { return energy_brick_fits(L); }
+
+hipError_t launch_energies_noalloc(const double *y, const ewpt_lattice &L, const Phys &P, double *out,
+                                   hipStream_t st)
+{
+    double *scr = nullptr;
+    hipError_t e = scratch_address(&scr);
+    if (e != hipSuccess) return e;
+    const Geo g = make_geo(L);
+    const long n3 = (long)L.nx * L.ny * L.nz, nb = n3 / EBT;
+    const int grid = (int)std::min<long>(nb, NOALLOC_BLOCKS);
+    double *part = scr, *res = scr + (size_t)NOALLOC_BLOCKS * NRED;
+    energy_brick_strided<<<grid, EBT, 0, st>>>(y, n3, g, P, nb, part);
+    finish_reduce<<<10, 256, 0, st>>>(part, grid, 1u << 8, res);
+    energy_table<<<1, 64, 0, st>>>(res, L.dx * L.dx * L.dx, (double)n3, out);
     return hipGetLastError();
 }
 

@@ -22,6 +22,13 @@ hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P
                            hipStream_t st);
 hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
                         hipStream_t st);
+// The energy table of launch_energies with no memory of the library's own:
+// partials in a fixed file-scope device array; the 13 columns (as
+// ewpt_energies returns them) written to the device buffer out.  Lattices
+// that tile 16x4x4 only.  Asynchronous on st.
+bool energies_noalloc_fits(const ewpt_lattice &L);
+hipError_t launch_energies_noalloc(const double *y, const ewpt_lattice &L, const Phys &P, double *out,
+                                   hipStream_t st);
 
 // EM diagnostics (em_point): fields of the groups in mask (bit 0 B, 1 B_2,
 // 2 A) to out; energy sums 0.5|B|^2, 0.5|B_2|^2, B_2.A (unscaled) to res[0..2],

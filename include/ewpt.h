@@ -211,6 +211,23 @@ int ewpt_slab_step_timed(ewpt_ctx *const *slabs, int n, const ewpt_params *p, in
  * Returns EWPT_ERR_NONFINITE (after filling out) if any density is NaN/Inf. */
 int ewpt_energies(ewpt_ctx *ctx, const ewpt_params *p, double out[13]);
 
+/* The energy table of ewpt_energies for a state in a caller's device buffer,
+ * with no allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  dev_state = 36*nx*ny*nz doubles
+ * and dev_out = 13 doubles, both on the current device; the 13 columns are
+ * those ewpt_energies returns (sums grouped differently: equal to rounding).
+ * Enqueued on `stream` (a hipStream_t of the current device, NULL = default
+ * stream), asynchronous, nothing allocated or freed.  The partial sums live
+ * in a fixed library array of 2048 x 16 doubles per device, whatever the
+ * lattice size; every call on a device shares it, so calls that overlap on
+ * one device race.  A non-finite density shows as a NaN/Inf column (no
+ * EWPT_ERR_NONFINITE: the call does not wait for the result).  Lattices that
+ * tile 16x4x4 (else EWPT_ERR_UNSUPPORTED); sizes as ewpt_create (else
+ * EWPT_ERR_INVAL). */
+int ewpt_energies_device(const ewpt_lattice *lat, const ewpt_params *p, const double *dev_state, double *dev_out,
+                         void *stream);
+
 /* Electromagnetic columns of the energy table, FP64:
  *   out[0] = column 8:  0.5 sum |B|^2 dx^3   (main_rk4.jl:5830-5868, 6113)
  *   out[1] = column 9:  0.5 sum |B_2|^2 dx^3 (5887-5926, 6114)

@@ -113,8 +113,12 @@ def test_exchange_primitives_as_a_multi_gpu_driver():
 
 
 def _hip():
-    """The HIP runtime libewpt is linked against (same loaded instance)."""
-    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    """The HIP runtime libewpt is linked against: opened by its soname after
+    the library, so the loader hands back that same instance (opening the
+    file by path would load it again against whichever HSA runtime the
+    process brought up first)."""
+    ewpt.load()
+    hip = ctypes.CDLL("libamdhip64.so.7")
     hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     hip.hipFree.argtypes = [ctypes.c_void_p]
     hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
