@@ -659,6 +659,42 @@ int ewpt_init_thermal(ewpt_ctx *ctx, double T, double meff_sq, unsigned long lon
     return EWPT_OK;
 }
 
+int ewpt_init_thermal_gauss(ewpt_ctx *ctx, double T, double meff_sq, unsigned long long seed, const ewpt_params *p,
+                            double stats[4])
+{
+    if (!ctx || !std::isfinite(T) || !(T > 0.0) || !std::isfinite(meff_sq) || !params_ok(p)) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "init_thermal_gauss");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    // free between steps: the stored derivatives and the next-state arrays
+    void *work;
+    double *wa, *wb;
+    if (mixed(ctx)) {
+        work = ctx->fk[0];                                  // 20 N^3 floats >= 3 N^3 complex doubles
+        wa = reinterpret_cast<double *>(ctx->fhi_n);       // 36 N^3 floats >= 15 N^3 doubles
+        wb = reinterpret_cast<double *>(ctx->flo_n);       // 36 N^3 floats >= 9 N^3 doubles
+    } else {
+        work = ctx->k[0];
+        wa = ctx->ynext;
+        wb = ctx->k[1];
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->y, 0, NCH * ctx->n3 * sizeof(double), ctx->stream), "init_thermal_gauss clear");
+    double st[4] = {0.0, 0.0, 0.0, 0.0};
+    const char *what = "";
+    if (thermal_gauss_state(ctx->y, ctx->lat, T, meff_sq, seed, p->gw, p->gy, 1e-13, 4000, work, wa, wb, ctx->stream,
+                            st, &what) != 0) {
+        (void)hipGetLastError();
+        return fail(ctx, EWPT_ERR_HIP, what);
+    }
+    if ((rc = push_view(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "init_thermal_gauss sync");
+    ctx->has_state = true;
+    if (stats)
+        for (int q = 0; q < 4; q++) stats[q] = st[q];
+    if (!std::isfinite(st[1])) return fail(ctx, EWPT_ERR_NONFINITE, "Gauss-law solve did not converge");
+    return EWPT_OK;
+}
+
 int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p, const double *din, double *dout, void *stream)
 {
     if (!lattice_ok(lat) || !params_ok(p) || !din || !dout || din == dout) return EWPT_ERR_INVAL;

@@ -96,6 +96,13 @@ hipError_t launch_add_bubbles(double *y, const ewpt_lattice &L, const double *db
 // returns 0, or -1 with *what naming the failed call; work holds 3*N^3 complex doubles
 int thermal_state(double *y, const ewpt_lattice &L, double T, double meff_sq, unsigned long long seed, int profile,
                   void *work, hipStream_t st, const char **what);
+// thermal_state (signed modes, Gamma = div W, Sigma = div Y) plus thermal E
+// and pi, with E projected onto the Gauss law (FP64 CG + FFT).  work: 3 N^3
+// complex doubles; wa: 15 N^3 doubles; wb: 9 N^3 doubles.  stats: residual
+// norm before, after, CG iterations, SU(2) part after.  0 or -1 (*what).
+int thermal_gauss_state(double *y, const ewpt_lattice &L, double T, double meff_sq, unsigned long long seed,
+                        double gw, double gy, double tol, int maxit, void *work, double *wa, double *wb,
+                        hipStream_t st, double stats[4], const char **what);
 
 // ewpt_batch.hip -- one persistent launch per stage over the bricks of many
 // lattices.  BatchLat describes one lattice: its stage buffers for parity 0

@@ -41,7 +41,7 @@ EXPORTS = (
     "ewpt_net_set_input", "ewpt_copy_probe", "ewpt_create_slab", "ewpt_slab_halo_bytes", "ewpt_slab_buffers",
     "ewpt_slab_pack", "ewpt_slab_unpack", "ewpt_stage", "ewpt_get_stream", "ewpt_sync", "ewpt_slab_step",
     "ewpt_slab_step_timed", "ewpt_set_state_device", "ewpt_get_state_device",
-    "ewpt_energies_device",
+    "ewpt_energies_device", "ewpt_init_thermal_gauss",
 )
 
 
@@ -100,6 +100,7 @@ def load() -> ctypes.CDLL:
                             ctypes.c_double, dp],
         "ewpt_init_bubbles": [vp, dp, ip, ip, ctypes.c_double],
         "ewpt_init_thermal": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, ip],
+        "ewpt_init_thermal_gauss": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, pp, dp],
         "ewpt_em_energies": [vp, pp, ctypes.c_double, dp],
         "ewpt_em_fields": [vp, pp, ctypes.c_double, dp],
         "ewpt_y_slice": [vp, pp, ctypes.c_double, ip, dp],

@@ -115,6 +115,16 @@ class Lattice:
         self._check(self._L.ewpt_init_thermal(self._h, T, meff_sq, ctypes.c_ulonglong(seed), profile_id(profile)),
                     "ewpt_init_thermal")
 
+    def init_thermal_gauss(self, T: float, meff_sq: float, seed: int, p) -> dict:
+        """Thermal fields (profile "config") with thermal E and pi projected
+        onto the Gauss law for the couplings of p; returns the residual norms
+        before and after the projection and the CG iteration count."""
+        st = np.zeros(4)
+        P = as_params(p)
+        self._check(self._L.ewpt_init_thermal_gauss(self._h, T, meff_sq, ctypes.c_ulonglong(seed), ctypes.byref(P),
+                                                    _dptr(st)), "ewpt_init_thermal_gauss")
+        return {"residual_before": st[0], "residual_after": st[1], "iterations": int(st[2]), "su2_after": st[3]}
+
     def copy_from(self, other: "Lattice"):
         """Device-to-device copy of other's state (same lattice and precision)."""
         self._check(self._L.ewpt_copy_state(self._h, other._h), "ewpt_copy_state")

@@ -130,6 +130,27 @@ int ewpt_init_bubbles(ewpt_ctx *ctx, const double *bubbles, int nb, int profile,
  * Needs T > 0 and finite meff_sq. */
 int ewpt_init_thermal(ewpt_ctx *ctx, double T, double meff_sq, unsigned long long seed, int profile);
 
+/* Thermal initial condition with thermal momenta that satisfy the Gauss
+ * law: the configuration's "Gauss law is solved for the initial electric
+ * fields".  No counterpart in the reference, whose thermal runs start from
+ * pi = E = 0 and drive the Higgs with J_ext instead (main_rk4.jl:5154-5162).
+ * Builds the fields of ewpt_init_thermal profile 1 (signed mode index,
+ * Gamma = div W, Sigma = div Y), then E^a, E^Y (transverse modes) and pi
+ * with Bose-Einstein amplitudes sqrt(n omega / vol), and projects them, in
+ * FP64, onto the Gauss law with the charges k_Gamma / k_Sigma weight by gp2
+ * (main_rk4.jl:603-657):
+ *   d_i E^a_i + gw eps^{abc} W^b_i E^c_i = gw Q^a(phi, pi),  d_i E^Y_i = gy Q^Y(phi, pi)
+ * (d the 6th-order first derivative): pi is corrected on the lattice modes
+ * d annihilates so the U(1) charge has no component there, E^Y gains a
+ * gradient from an exact FFT solve, E^a a covariant gradient from a
+ * conjugate-gradient solve of D^T D chi = G E0 - gw Q (FFT-preconditioned,
+ * relative residual 1e-13).  p supplies gw and gy.  stats (if not NULL):
+ * [0] residual norm sqrt(sum r^2 dx^3) over G^1..3, G^Y before the
+ * projection, [1] after it, [2] CG iterations, [3] the SU(2) part of [1].
+ * Needs T > 0; returns EWPT_ERR_NONFINITE if the solve produced NaN/Inf. */
+int ewpt_init_thermal_gauss(ewpt_ctx *ctx, double T, double meff_sq, unsigned long long seed, const ewpt_params *p,
+                            double stats[4]);
+
 /* One RHS evaluation, host buffers: the @cuda rk4_step!(...) launch
  * (main_rk4.jl:52-661; launch 6155).  in = 36-channel stage state, out =
  * the 36 k/kt arrays in the same channel order (0-15 k = momenta copies,
