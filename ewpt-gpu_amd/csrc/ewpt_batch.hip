@@ -99,6 +99,7 @@ struct Kern {
     static size_t lds(int stage)
     {
         const size_t l13 = 2 * B::BUF * sizeof(T);
+        if (stage == 1 && park_centres(PF, 1)) return l13 + NK * B::NT * sizeof(T);
         return stage < 4 ? l13 : l13 + (std::is_same_v<T, float> && !lean_stash(PF) ? 2 : 1) * NK * B::NT * sizeof(T);
     }
     template <bool BETA, int S, bool PS>
@@ -198,7 +199,7 @@ auto with_kern(int v, F &&f)
     if constexpr (std::is_same_v<T, float>) {
         if (v == 15) return f(Kern<32, 4, 4, 704432, T>{});
         if (v == 31) return f(Kern<32, 4, 4, 1010704443, T>{});
-        if (v == 33) return f(Kern<32, 4, 4, 1210704443, T>{});
+        if (v == 33) return f(Kern<32, 4, 4, 1220704443, T>{});
     }
     if constexpr (std::is_same_v<T, double>) {
         if (v == 2) return f(Kern<8, 8, 8, 1, T>{});

@@ -280,8 +280,22 @@ __host__ __device__ constexpr bool stored_in(int PF, int S) { return S >= 2 && (
 __host__ __device__ constexpr bool late_centres(int PF) { return (PF / 1000000) % 10 == 1; }
 __host__ __device__ constexpr bool is_late_channel(int ch) { return (ch >= 16 && ch < 20) || ch >= 24; }
 // PF / 10000000 = 1: stages 2-3 that write the next stage input load the y
-// values it needs during their last passes instead of after them.
-__host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10 == 1; }
+// values it needs during their last passes instead of after them.  = 2: the
+// same, and the 16 field channels of that input (which need no derivative
+// of this stage) are stored as soon as their y arrives, so only the other 20
+// y values are held in registers to the end (fewer live registers: the FP32
+// kernels at four waves per SIMD stop spilling; the FP64 kernels, which do
+// not spill, measured slower with the stores during the passes).
+__host__ __device__ constexpr bool y_early(int PF) { return (PF / 10000000) % 10 == 1 || (PF / 10000000) % 10 == 2; }
+__host__ __device__ constexpr bool early_fields(int PF) { return (PF / 10000000) % 10 == 2; }
+// Stage 1 of an early_fields variant also parks the 20 centre values of
+// channels 16..35 (E, Gamma/Sigma, pi) in LDS once their passes have staged
+// them: after that only the epilogue (the next stage input y + c k_1) reads
+// them, so their registers are free for the phi, Y and W passes.
+__host__ __device__ constexpr bool park_centres(int PF, int S)
+{
+    return early_fields(PF) && S == 1 && stored_in(PF, 2) && !late_centres(PF);
+}
 // PF / 100000000 = 1 or 2: stage 4 loads y, k1..k3 for the partial sums of
 // channels 16..35 (mixed: the k's and the compensation word for the
 // increments) one channel per pass during its first 20 passes instead of all
@@ -328,6 +342,8 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     R *lds_base = reinterpret_cast<R *>(lds_raw);
     R *lds[4] = {lds_base, lds_base + B::BUF, lds_base + 2 * B::BUF, lds_base + 3 * B::BUF};   // [2..3]: paired
     R *nstash = lds_base + 2 * B::BUF;
+    R *park = lds_base + 2 * B::BUF;   // stage 1 with park_centres: [NK][NT]
+    constexpr bool PARK = park_centres(PF, S);
 
     // ---- brick of this workgroup
     int bx, by, bz;
@@ -457,6 +473,11 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         for (int i = 0; i < 3; i++) ktY[i] = R(0);
     }
 
+    if constexpr (PARK) {
+#pragma unroll
+        for (int ch = CH_PI; ch < CH_PI + 4; ch++) park[(ch - 16) * B::NT + t] = c[ch];   // never staged
+    }
+
     // ---- staging of one channel into LDS: centre from registers, faces from
     // global.  Per face class (0 = star, 1/2/3 = x/y/z faces) each thread owns
     // up to MAXF halo elements; their in-channel byte offsets and LDS offsets
@@ -543,6 +564,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             buf[lcen] = cv;
         } else {
             buf[lcen] = c[pass_channel(pp)];
+            if constexpr (PARK && pass_channel(pp) >= 16) park[(pass_channel(pp) - 16) * B::NT + t] = c[pass_channel(pp)];
         }
 #pragma unroll
         for (int s = 0; s < MAXF; s++)
@@ -736,8 +758,13 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     [&]<int... I>(std::integer_sequence<int, I...>) {
         (fetch(std::integral_constant<int, I>{}, rr[I]), ...);
     }(std::make_integer_sequence<int, PFS>{});
-    // y for the next stage input, loaded during the last YP passes (y_early)
+    // y for the next stage input, loaded during the last YP passes (y_early).
+    // With early_fields the 16 field channels of that input, y + c * (this
+    // stage's momentum input), are stored as soon as their y arrives (passes
+    // NPASS-YP .. NPASS-YP/2-1); the other 20 need this stage's derivatives,
+    // so their y is held until the end (loaded in the last YP/2 passes).
     constexpr bool YE = y_early(PF) && S > 1 && stores_next(PF, S) && !paired_of(PF, S);
+    constexpr bool YF = YE && early_fields(PF);
     constexpr int YP = 8;
     R ypre[YE ? NCH : 1];
     if constexpr (!paired_of(PF, S) && !SPREAD) {
@@ -745,10 +772,23 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         lds_barrier();
         auto body = [&](auto pc) {
             constexpr int pp = decltype(pc)::value;
-            if constexpr (YE && pp >= NPASS - YP) {
+            if constexpr (YE && !YF && pp >= NPASS - YP) {
                 constexpr int q = pp - (NPASS - YP);
 #pragma unroll
                 for (int ch = q * NCH / YP; ch < (q + 1) * NCH / YP; ch++) ypre[ch] = ldo(Y(ch), qb);
+            }
+            if constexpr (YF && pp >= NPASS - YP) {
+                constexpr int q = pp - (NPASS - YP);
+                if constexpr (q < YP / 2) {
+#pragma unroll
+                    for (int ch = q * 16 / (YP / 2); ch < (q + 1) * 16 / (YP / 2); ch++)
+                        stn(b.unext + (long)ch * b.n3, qb, R(ldo(Y(ch), qb)) + c[ch + 20] * (S == 3 ? dt : half_dt));
+                } else {
+                    constexpr int q2 = q - YP / 2;
+#pragma unroll
+                    for (int ch = 16 + q2 * 20 / (YP / 2); ch < 16 + (q2 + 1) * 20 / (YP / 2); ch++)
+                        ypre[ch] = ldo(Y(ch), qb);
+                }
             }
             if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
             consume(pc, lds[pp & 1]);
@@ -877,9 +917,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             // the other channels y + c * k_s (the expressions of stage_in)
             const R cc = S == 3 ? dt : half_dt;
 #pragma unroll
-            for (int ch = 0; ch < NCH; ch++) {
+            for (int ch = YF ? 16 : 0; ch < NCH; ch++) {   // early_fields: the field channels are stored
                 R yv;
-                if constexpr (S == 1) yv = c[ch];
+                if constexpr (PARK) yv = ch >= 16 ? park[(ch - 16) * B::NT + t] : c[ch];
+                else if constexpr (S == 1) yv = c[ch];
                 else if constexpr (YE) yv = ypre[ch];
                 else yv = R(ldo(Y(ch), qb));
                 const R kv = ch < 16 ? c[ch + 20] : o[ch - 16];
@@ -930,7 +971,7 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
 {
     using B = BrickShape<TX, TY, TZ>;
     const dim3 grid(g.nbx * g.nby * g.nbz), block(B::NT);
-    const size_t lds13 = (paired_of(PF, 1) ? 4 : 2) * B::BUF * sizeof(T);
+    const size_t lds13 = (paired_of(PF, 1) ? 4 : 2) * B::BUF * sizeof(T) + (park_centres(PF, 1) ? NK * B::NT * sizeof(T) : 0);
     // stage 4 (never paired) parks n (FP64) or the increment and the high word (mixed)
     const size_t lds4 =
         2 * B::BUF * sizeof(T) + (std::is_same_v<T, float> && !lean_stash(PF) ? 2 : 1) * NK * B::NT * sizeof(T);

@@ -12,8 +12,10 @@
 //                    in LDS (two workgroups per CU) and prefetch depth 4/4/4/3;
 //                    the same arithmetic in the same order as 15
 //   33 (+100*order): 31 with stage 4's increments formed one channel per pass
-//                    and the field channels updated from the same k loads
-//                    (bitwise equal to 31)
+//                    and the field channels updated from the same k loads, and
+//                    stages 2-3 storing the field channels of the next input
+//                    during their last passes (no register spills at four
+//                    waves per SIMD); bitwise equal to 31
 #include "ewpt_brick.h"
 
 namespace ewpt {
@@ -49,7 +51,7 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     case 24: return launch_tile<16, 4, 4, 1704432, float>(stage, b, L, P, dt, 1, order, st);
     case 25: return launch_tile<16, 4, 4, 1004432, float>(stage, b, L, P, dt, 1, order, st);
     case 31: return launch_tile<32, 4, 4, 1010704443, float>(stage, b, L, P, dt, 1, order, st);
-    case 33: return launch_tile<32, 4, 4, 1210704443, float>(stage, b, L, P, dt, 1, order, st);
+    case 33: return launch_tile<32, 4, 4, 1220704443, float>(stage, b, L, P, dt, 1, order, st);
     default: return hipErrorInvalidValue;
     }
 }
