@@ -478,7 +478,7 @@ def main(argv=None) -> int:
     params = ewpt.Parameters(N=n, dx=dx, gy=0.35, T=T)
     P = params.step_params()
     sites = n ** 3
-    total_steps = max(a.precision_steps, a.warmup + a.steps)
+    total_steps = max(a.precision_steps, a.warmup + a.steps + 8)   # + the 8 steps of the snapshot timing
     try:
         import torch
         dev_ok = torch.cuda.is_available()
@@ -506,7 +506,23 @@ def main(argv=None) -> int:
             stage_ms, total_ms = lat.step_timed(P, a.steps)   # synchronises before returning
             active = lat.active_variant
             e_timed = lat.energies(P)
-            done = a.warmup + a.steps
+            # snapshot cost: a step that also reduces the energy table and Gauss
+            # norms of its input (fused into stage 1 where the variant has it)
+            # against a plain step and the standalone reductions (wall clock,
+            # each call synchronous; mean of 3 after one warm call)
+            def wall(f, n=3):
+                f()
+                t0 = time.perf_counter()
+                for _ in range(n):
+                    f()
+                return (time.perf_counter() - t0) / n * 1e3
+            snap = {"step_ms": wall(lambda: lat.step(P, 1)),
+                    "step_diag_ms": wall(lambda: lat.step_diag(P, 1)),
+                    "standalone_energies_gauss_ms": wall(lambda: (lat.energies(P), lat.gauss(P)))}
+            snap["fused_overhead_frac"] = snap["step_diag_ms"] / snap["step_ms"] - 1.0
+            snap["fused"] = precision == "fp64" and active % 100 == 32
+            done_snap = 8             # wall() runs each stepping call 1 + 3 times
+            done = a.warmup + a.steps + done_snap
             if a.precision_steps:
                 t0 = time.perf_counter()
                 while done < total_steps:
@@ -549,6 +565,7 @@ def main(argv=None) -> int:
             # the step against the minimal (formed-input) traffic of an RK4 step
             "step_roofline_frac": value * sum(STAGE_BYTES[precision].values()) / HBM_PEAK,
             "stage_ms": stage_ms,
+            "snapshot": snap,
             "energy_check": float(sum(e_timed[:7])),
             "steps_run": done,
             "energy_drift": float((sum(e1[:7]) - sum(e0[:7])) / sum(e0[:7])),
@@ -586,6 +603,7 @@ def main(argv=None) -> int:
         "roofline": head["roofline"],
         "step_roofline_frac": head["step_roofline_frac"],
         "stage_ms": head["stage_ms"],
+        "snapshot": head["snapshot"],
         "energy_check": head["energy_check"],
     }
     if "mixed" in res and precs[0] != "mixed":
@@ -593,7 +611,7 @@ def main(argv=None) -> int:
         out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"], "variant": m["variant"],
                         "speedup_vs_fp64": m["value"] / head["value"], "dtype": "f32 storage/stencils, f64 reductions",
                         "roofline": m["roofline"], "step_roofline_frac": m["step_roofline_frac"],
-                        "stage_ms": m["stage_ms"], "energy_check": m["energy_check"],
+                        "stage_ms": m["stage_ms"], "snapshot": m["snapshot"], "energy_check": m["energy_check"],
                         "energy_rel_diff_vs_fp64": abs(m["energy_check"] - head["energy_check"]) / abs(head["energy_check"])}
     if len(finals) == 2:
         ref, mix = finals["fp64"], finals["mixed"]

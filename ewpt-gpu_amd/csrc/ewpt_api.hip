@@ -52,6 +52,9 @@ struct ewpt_ctx {
     // reductions
     double *part = nullptr, *res = nullptr;
     long nred = 0;
+    // partial sums of the diagnostic stage 1 (rows x DIAG_SLOTS, then the
+    // DIAG_SLOTS reduced values), allocated on first use
+    double *diag = nullptr;
     bool has_state = false;
     unsigned long long bytes = 0;
     std::string err;
@@ -96,7 +99,7 @@ int dev_alloc(ewpt_ctx *c, T **p, size_t n, const char *what)
 void release(ewpt_ctx *c)
 {
     double **bufs[] = {&c->y, &c->ynext, &c->k[0], &c->k[1], &c->k[2], &c->rhs_in, &c->rhs_out, &c->part, &c->res,
-                       &c->u};
+                       &c->u, &c->diag};
     for (double **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     float **fbufs[] = {&c->fhi, &c->flo, &c->fhi_n, &c->flo_n, &c->fk[0], &c->fk[1], &c->fk[2], &c->fu};
@@ -728,6 +731,64 @@ int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps)
         for (int s = 1; s <= 4; s++)
             if ((rc = launch_step_stage(ctx, s, p))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "step sync");
+    return EWPT_OK;
+}
+
+namespace {
+
+// the energy table (ewpt_energies layout) and Gauss norms (ewpt_gauss layout)
+// from the DIAG_SLOTS reduced sums of the diagnostic stage 1
+void diag_outputs(const ewpt_ctx *c, const double *r, double e[13], double gs[8])
+{
+    const double vol = c->lat.dx * c->lat.dx * c->lat.dx;
+    for (int q = 0; q < 13; q++) e[q] = 0.0;
+    for (int q = 0; q < 7; q++) e[q] = r[q] * vol;
+    e[9] = r[8];
+    e[11] = r[7] / (double)sites(c);
+    for (int q = 0; q < 8; q++) gs[q] = std::sqrt(r[10 + q] * vol);
+}
+
+}  // namespace
+
+int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energies[13], double gauss[8])
+{
+    if (!ctx || !params_ok(p) || nsteps < 1 || !energies || !gauss) return EWPT_ERR_INVAL;
+    if (ctx->zg) return no_slab(ctx, "step_diag");
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "step before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const int v = resolved_variant(ctx);
+    if (mixed(ctx) || !brick_has_diag(v)) {
+        // no diagnostic stage 1 for this precision / variant: the standalone
+        // reductions of the state, then the steps
+        const int re = ewpt_energies(ctx, p, energies);
+        if (re && re != EWPT_ERR_NONFINITE) return re;
+        const int rg = ewpt_gauss(ctx, p, gauss);
+        if (rg && rg != EWPT_ERR_NONFINITE) return rg;
+        if ((rc = ewpt_step(ctx, p, nsteps))) return rc;
+        return re ? re : rg;
+    }
+    const long rows = diag_rows(ctx->lat);
+    if (!ctx->diag && (rc = dev_alloc(ctx, &ctx->diag, (size_t)(rows + 1) * DIAG_SLOTS, "alloc diagnostics")))
+        return rc;
+    if ((rc = ensure_stage_input_buffer(ctx, v))) return rc;
+    double *res = ctx->diag + rows * DIAG_SLOTS;
+    HIPCHK(ctx, launch_stage1_diag(v, bufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
+                                   ctx->diag, ctx->stream),
+           "diagnostic stage launch");
+    HIPCHK(ctx, launch_finish_rows(ctx->diag, rows, DIAG_SLOTS, 1u << 8, res, ctx->stream), "diagnostic reduction");
+    double r[DIAG_SLOTS];
+    HIPCHK(ctx, hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, ctx->stream), "diagnostic copy");
+    for (int s = 2; s <= 4; s++)
+        if ((rc = launch_stage_kernel(ctx, s, p))) return rc;
+    for (int it = 1; it < nsteps; it++)
+        for (int s = 1; s <= 4; s++)
+            if ((rc = launch_stage_kernel(ctx, s, p))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "step sync");
+    diag_outputs(ctx, r, energies, gauss);
+    bool bad = r[9] != 0.0;
+    for (int q = 0; q < 8; q++) bad |= !std::isfinite(gauss[q]);
+    if (bad) return fail(ctx, EWPT_ERR_NONFINITE, "energy density or Gauss residual is NaN/Inf");
     return EWPT_OK;
 }
 

@@ -327,9 +327,28 @@ __host__ __device__ constexpr int waves_per_simd(bool f32, int PF, int S) { retu
 // lattice described by b and g, using the dynamic LDS at lds_raw; t is the
 // thread's index in the workgroup.  It is called by the single-lattice
 // kernel stage_brick and by the batched kernels stage_brick_batch.
-template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T>
+// Slots of the diagnostics a stage-1 brick can reduce on the way (DIAG):
+// the energy densities of the reference's compute_* kernels
+// (main_rk4.jl:1334-2301) and the Gauss-law residuals of the k_Gamma /
+// k_Sigma charges (main_rk4.jl:603-657), in the layout of the standalone
+// reductions: 0 PE, 1 KE, 2 GE, 3 EEW, 4 MEW, 5 EEY, 6 MEY, 7 sum|phi|,
+// 8 min|phi|, 9 non-finite count, 10-12 (G^a)^2, 13 (G^Y)^2, 14-16 (C^a)^2,
+// 17 (C^Y)^2.
+constexpr int NDIAG = 18;
+
+__device__ __forceinline__ double wave_reduce(double v, bool mn)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = mn ? fmin(v, o) : v + o;
+    }
+    return v;
+}
+
+template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T, bool DIAG = false>
 __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, int bid,
-                                           unsigned char *lds_raw, int t)
+                                           unsigned char *lds_raw, int t, double *diag = nullptr)
 {
     // T = float is the mixed-precision path: y is stored as an FP32 pair
     // (b.y = high word, b.ylo = compensation word), stages read the high
@@ -478,6 +497,33 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         for (int ch = CH_PI; ch < CH_PI + 4; ch++) park[(ch - 16) * B::NT + t] = c[ch];   // never staged
     }
 
+    // ---- diagnostics (DIAG, stage 1 of the FP64 path).  Every per-site
+    // value is parked in the thread's own LDS slots dgl[k][t] the moment it is
+    // known (the derivative sums div E^a, div E^Y, |D phi|^2, div Y, div W^a
+    // accumulate there beside the right-hand side's); after the passes, when
+    // the registers are free, each slot is wave-reduced (dgw[wave][slot]) and
+    // the waves combined in a fixed order.  Per-thread slots k:
+    //   0-3 div E^a, div E^Y, then (G^a)^2, (G^Y)^2   4 |D phi|^2
+    //   5 div Y, then (C^Y)^2   6-8 div W^a, then (C^a)^2
+    //   9 PE  10 KE  11 |phi|  12 EEW  13 EEY  14 MEY  15 MEW
+    constexpr bool DG = DIAG && S == 1;
+    static_assert(!DIAG || (std::is_same_v<T, double> && S == 1 && !PARK), "diagnostics: FP64 stage 1");
+    constexpr int NKD = 16;
+    double *dgw = reinterpret_cast<double *>(lds_raw + 2 * B::BUF * sizeof(T));
+    double *dgl = dgw + (B::NT / 64) * NDIAG;
+    auto dput = [&](int k, double v) { dgl[k * B::NT + t] = v; };
+    auto dacc = [&](int k, double v) { dgl[k * B::NT + t] += v; };
+    auto dget = [&](int k) { return dgl[k * B::NT + t]; };
+    if constexpr (DG) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) dput(k, 0.0);
+        const R X = r2 - R(P.vev) * R(P.vev);
+        dput(9, R(P.lam) * (X * X));
+        dput(10, c[CH_PI] * c[CH_PI] + c[CH_PI + 1] * c[CH_PI + 1] + c[CH_PI + 2] * c[CH_PI + 2] +
+                     c[CH_PI + 3] * c[CH_PI + 3]);
+        dput(11, sqrt(r2));
+    }
+
     // ---- staging of one channel into LDS: centre from registers, faces from
     // global.  Per face class (0 = star, 1/2/3 = x/y/z faces) each thread owns
     // up to MAXF halo elements; their in-channel byte offsets and LDS offsets
@@ -594,6 +640,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             deriv(buf, i, cen, false, d1, d2);
             if constexpr (a < 3) kG[a] += (R(1) - gp2) * d1;
             else kS += (R(1) - gp2) * d1;
+            if constexpr (DG) dacc(a, d1);
             if constexpr (late_centres(PF) && pp == 11) {
                 // every E centre is staged: the W x E term of k_Gamma
 #pragma unroll
@@ -604,6 +651,26 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                         na -= xprod(aa, W(0, ii), W(1, ii), W(2, ii), E(0, ii), E(1, ii), E(2, ii));
                     kG[aa] += gp2 * gw * na;
                 }
+            }
+            if constexpr (DG && pp == 11) {
+                // every E centre and div E is known: EEW, EEY and the Gauss residuals
+                R ee = R(0);
+#pragma unroll
+                for (int qq = 0; qq < 9; qq++) ee += c[CH_E + qq] * c[CH_E + qq];
+                dput(12, R(0.5) * ee);
+                dput(13, R(0.5) * (c[CH_EY] * c[CH_EY] + c[CH_EY + 1] * c[CH_EY + 1] + c[CH_EY + 2] * c[CH_EY + 2]));
+                const R piv[4] = {c[CH_PI], c[CH_PI + 1], c[CH_PI + 2], c[CH_PI + 3]};
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) {
+                    R na = R(0);
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++)
+                        na -= xprod(aa, W(0, ii), W(1, ii), W(2, ii), E(0, ii), E(1, ii), E(2, ii));
+                    const R r = dget(aa) - gw * (na + phiT(aa, p, piv));
+                    dput(aa, r * r);
+                }
+                const R rY = dget(3) - gy * phiTY(p, piv);
+                dput(3, rY * rY);
             }
         } else if constexpr (kind == 1) {
             // Gamma^a, Sigma: -d_i Gamma^a into kt_W^a_i, -d_i Sigma into kt_Y_i
@@ -639,6 +706,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 R Ap[4];
                 apply_gen(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], p, Ap);
                 const R D = d1 + Ap[bb];
+                if constexpr (DG) dacc(4, D * D);
                 add_gen_col<bb>(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], d1 + D, ktphi);
                 ktW[0][i] += gw * phiT_coef<0, bb>(p) * D;
                 ktW[1][i] += gw * phiT_coef<1, bb>(p) * D;
@@ -656,14 +724,21 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 R d1, d2;
                 deriv(buf, j, cen, true, d1, d2);
                 lap += d2;
-                if constexpr (BETA) {
+                if constexpr (BETA || DG) {
                     if (j != i) {
                         if (i == (j + 1) % 3) BYv[(j + 2) % 3] += d1;
                         else BYv[(j + 1) % 3] -= d1;
                     }
                 }
+                if constexpr (DG)
+                    if (j == i) dacc(5, d1);
             }
             ktY[i] += lap;
+            if constexpr (DG && pp == 22) {
+                dput(14, R(0.5) * (BYv[0] * BYv[0] + BYv[1] * BYv[1] + BYv[2] * BYv[2]));
+                const R cY = c[CH_SIG] - dget(5);
+                dput(5, cY * cY);
+            }
             if constexpr (BETA && pp == 22) {
                 // B^Y complete: beta_Y terms (main_rk4.jl:354, 553-581)
                 const R QY = c[CH_EY] * BYv[0] + c[CH_EY + 1] * BYv[1] + c[CH_EY + 2] * BYv[2];
@@ -697,12 +772,25 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 constexpr int a1 = (a + 1) % 3, a2 = (a + 2) % 3;   // a' values
                 ktW[a1][i] -= gw * R(eps3(a1, a)) * W(3 - a1 - a, j) * d1;
                 ktW[a2][i] -= gw * R(eps3(a2, a)) * W(3 - a2 - a, j) * d1;
+                if constexpr (DG)
+                    if (j == i) dacc(6 + a, d1);
                 if (j != i) {
                     if (i == (j + 1) % 3) Bw[a][(j + 2) % 3] += d1;
                     else Bw[a][(j + 1) % 3] -= d1;
                 }
             }
             ktW[a][i] += lap;
+            if constexpr (DG && pp == 31) {
+                R me = R(0);
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) me += Bw[aa][0] * Bw[aa][0] + Bw[aa][1] * Bw[aa][1] + Bw[aa][2] * Bw[aa][2];
+                dput(15, R(0.5) * me);
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) {
+                    const R cc = c[CH_GAM + aa] - dget(6 + aa);
+                    dput(6 + aa, cc * cc);
+                }
+            }
             if constexpr (pp == 31) {
                 // B^a complete: -gw sum_j (W_j x F_ij)^a, beta_W terms
                 if constexpr (BETA) {
@@ -739,7 +827,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         for (int i = 0; i < 3; i++) {
             pin(ktY[i]); pin(kG[i]);
             if constexpr (BETA && pp >= 16) pin(u[i]);
-            if constexpr (BETA && pp >= 20 && pp < 22) pin(BYv[i]);
+            if constexpr ((BETA || DG) && pp >= 20 && pp < 22) pin(BYv[i]);
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 pin(ktW[i][j]);
@@ -940,6 +1028,32 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 #pragma unroll
         for (int m = 0; m < NK; m++) stn(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
     }
+    if constexpr (DG) {
+        // the brick's partial sums: per-thread slots wave-reduced, then the
+        // waves combined in a fixed order
+        constexpr int slot_of[NKD] = {10, 11, 12, 13, 2, 17, 14, 15, 16, 0, 1, 7, 3, 5, 6, 4};
+        int bad = 0;
+#pragma unroll
+        for (int k = 0; k < NKD; k++) {
+            const double v = dget(k);
+            if (slot_of[k] < 8) bad += !isfinite(v);
+            const double r = wave_reduce(v, false);
+            if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + slot_of[k]] = r;
+            if (k == 11) {
+                const double rm = wave_reduce(v, true);
+                if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + 8] = rm;
+            }
+        }
+        const double rb = wave_reduce(double(bad), false);
+        if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + 9] = rb;
+        lds_barrier();
+        if (t < NDIAG) {
+            double acc = t == 8 ? INFINITY : 0.0;
+#pragma unroll
+            for (int w = 0; w < B::NT / 64; w++) acc = t == 8 ? fmin(acc, dgw[w * NDIAG + t]) : acc + dgw[w * NDIAG + t];
+            diag[(long)bid * NDIAG + t] = acc;
+        }
+    }
 }
 
 template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T = double>
@@ -956,6 +1070,21 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
         bid = (bid & 7) * per + (bid >> 3);
     }
     brick_body<TX, TY, TZ, PF, BETA, S, T>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x);
+}
+
+// Stage 1 with the diagnostics of its input (the state) reduced on the way:
+// per brick NDIAG partial sums to diag[brick * NDIAG + slot].
+template <int TX, int TY, int TZ, int PF, bool BETA>
+__global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(waves_per_simd(false, PF, 1))))
+stage1_diag(StageBufs<double> b, BrickGeo g, Phys P, double dt, double *diag)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    int bid = blockIdx.x;
+    if (g.xcd_map) {
+        const int per = (g.nbx * g.nby * g.nbz) >> 3;
+        bid = (bid & 7) * per + (bid >> 3);
+    }
+    brick_body<TX, TY, TZ, PF, BETA, 1, double, true>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag);
 }
 
 template <int TX, int TY, int TZ>
@@ -1001,6 +1130,51 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
     default: stage_brick<TX, TY, TZ, PF, BETA, 4, T><<<grid, block, lds4, st>>>(b, g, P, dt); break;
     }
     return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, int PF, bool BETA>
+hipError_t launch_diag_beta(const StageBufs<double> &b, const BrickGeo &g, const Phys &P, double dt, double *diag,
+                            hipStream_t st)
+{
+    using B = BrickShape<TX, TY, TZ>;
+    const size_t lds = 2 * B::BUF * sizeof(double) + ((B::NT / 64) * NDIAG + 16 * B::NT) * sizeof(double);
+    static std::mutex attr_mu;
+    static bool attr_set = false;
+    {
+        std::lock_guard<std::mutex> gd(attr_mu);
+        if (!attr_set) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage1_diag<TX, TY, TZ, PF, BETA>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
+    }
+    stage1_diag<TX, TY, TZ, PF, BETA><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag);
+    return hipGetLastError();
+}
+
+BrickGeo make_brick_geo(int TX, int TY, int TZ, const ewpt_lattice &L, int xcd_map, int order, int zg)
+{
+    BrickGeo g;
+    g.order = order;
+    g.nx = L.nx; g.ny = L.ny; g.nz = L.nz;
+    g.nbx = L.nx / TX; g.nby = L.ny / TY; g.nbz = L.nz / TZ;
+    const int nb = g.nbx * g.nby * g.nbz;
+    g.xcd_map = (xcd_map && nb % 8 == 0) ? 1 : 0;
+    g.inv60dx = 1.0 / (60.0 * L.dx);
+    g.invdx2 = 1.0 / (L.dx * L.dx);
+    g.zg = zg;
+    return g;
+}
+
+// stage 1 with the diagnostics (diag: one NDIAG row per brick)
+template <int TX, int TY, int TZ, int PF>
+hipError_t launch_tile_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt, int xcd_map,
+                            int order, double *diag, hipStream_t st)
+{
+    const BrickGeo g = make_brick_geo(TX, TY, TZ, L, xcd_map, order, b.zghost);
+    if (P.bw != 0.0 || P.by != 0.0) return launch_diag_beta<TX, TY, TZ, PF, true>(b, g, P, dt, diag, st);
+    return launch_diag_beta<TX, TY, TZ, PF, false>(b, g, P, dt, diag, st);
 }
 
 template <int TX, int TY, int TZ, int PF, typename T = double>

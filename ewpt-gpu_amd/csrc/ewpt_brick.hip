@@ -33,6 +33,8 @@ EWPT_DECLARE_BRICK(24)
 EWPT_DECLARE_BRICK(25)
 EWPT_DECLARE_BRICK(32)
 #undef EWPT_DECLARE_BRICK
+hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                                 int order, double *diag, hipStream_t st);
 
 namespace {
 
@@ -72,6 +74,20 @@ bool brick_supported(int variant, const ewpt_lattice &L)
 {
     const BrickEntry *e = find_brick(variant);
     return e && e->fits(L);
+}
+
+long diag_rows(const ewpt_lattice &L)
+{
+    return (long)(L.nx / 16) * (L.ny / 4) * (L.nz / 4);   // the 16x4x4 bricks of variant 32
+}
+
+bool brick_has_diag(int variant) { return variant >= 0 && variant % 100 == 32 && brick_exists(variant); }
+
+hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                              double *diag, hipStream_t st)
+{
+    if (!brick_has_diag(variant)) return hipErrorInvalidValue;
+    return launch_brick_v32_diag(b, L, P, dt, variant / 100, diag, st);
 }
 
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,

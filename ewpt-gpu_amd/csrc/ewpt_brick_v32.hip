@@ -15,4 +15,10 @@ hipError_t launch_brick_v32(int stage, const StageBufs<double> &b, const ewpt_la
     return launch_tile<16, 4, 4, 211704432>(stage, b, L, P, dt, 1, order, st);
 }
 
+hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                                 int order, double *diag, hipStream_t st)
+{
+    return launch_tile_diag<16, 4, 4, 211704432>(b, L, P, dt, 1, order, diag, st);
+}
+
 }  // namespace ewpt

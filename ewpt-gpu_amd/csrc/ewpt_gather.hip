@@ -733,6 +733,36 @@ hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, d
     return hipGetLastError();
 }
 
+namespace {
+
+__global__ void __launch_bounds__(256) finish_rows(const double *part, long nblk, int nslot, unsigned minmask,
+                                                  double *res)
+{
+    __shared__ double sh[256];
+    const int q = blockIdx.x;
+    const bool mn = (minmask >> q) & 1u;
+    double acc = mn ? INFINITY : 0.0;
+    for (long b = threadIdx.x; b < nblk; b += 256) {
+        const double v = part[b * nslot + q];
+        acc = mn ? fmin(acc, v) : acc + v;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = mn ? fmin(sh[threadIdx.x], sh[threadIdx.x + st]) : sh[threadIdx.x] + sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) res[q] = sh[0];
+}
+
+}  // namespace
+
+hipError_t launch_finish_rows(const double *part, long nblk, int nslot, unsigned minmask, double *res, hipStream_t st)
+{
+    finish_rows<<<nslot, 256, 0, st>>>(part, nblk, nslot, minmask, res);
+    return hipGetLastError();
+}
+
 hipError_t launch_em_fields(const double *y, const ewpt_lattice &L, const Phys &P, double theta_w, unsigned mask,
                             double *out, hipStream_t st)
 {

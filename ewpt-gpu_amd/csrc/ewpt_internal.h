@@ -57,6 +57,17 @@ bool brick_stored_inputs(int variant);
 bool brick_supported(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
                               const Phys &P, double dt, hipStream_t st);
+// stage 1 that also reduces the energy densities and Gauss residuals of its
+// input (the state): one row of DIAG_SLOTS partial sums per brick in diag
+// (slots: PE KE GE EEW MEW EEY MEY sum|phi| min|phi| non-finite, then
+// (G^a)^2 (G^Y)^2 (C^a)^2 (C^Y)^2).  FP64 variant 32 (+100*order) only.
+constexpr int DIAG_SLOTS = 18;
+bool brick_has_diag(int variant);
+long diag_rows(const ewpt_lattice &L);   // rows of diag (bricks of the diagnostic stage 1)
+hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
+                              double *diag, hipStream_t st);
+// sum (slot 8: min) nblk rows of a row-major [nblk][nslot] array, slot by slot in a fixed order
+hipError_t launch_finish_rows(const double *part, long nblk, int nslot, unsigned minmask, double *res, hipStream_t st);
 
 // ewpt_brick_f32.hip -- mixed-precision stage kernels
 bool brick_exists_f32(int variant);

@@ -41,7 +41,7 @@ EXPORTS = (
     "ewpt_net_set_input", "ewpt_copy_probe", "ewpt_create_slab", "ewpt_slab_halo_bytes", "ewpt_slab_buffers",
     "ewpt_slab_pack", "ewpt_slab_unpack", "ewpt_stage", "ewpt_get_stream", "ewpt_sync", "ewpt_slab_step",
     "ewpt_slab_step_timed", "ewpt_set_state_device", "ewpt_get_state_device",
-    "ewpt_energies_device", "ewpt_init_thermal_gauss",
+    "ewpt_energies_device", "ewpt_init_thermal_gauss", "ewpt_step_diag",
 )
 
 
@@ -90,6 +90,7 @@ def load() -> ctypes.CDLL:
         "ewpt_rhs_device": [lp, pp, vp, vp, vp],
         "ewpt_step": [vp, pp, ip],
         "ewpt_step_timed": [vp, pp, ip, dp, dp],
+        "ewpt_step_diag": [vp, pp, ip, dp, dp],
         "ewpt_energies": [vp, pp, dp],
         "ewpt_gauss": [vp, pp, dp],
         "ewpt_set_variant": [vp, ip],

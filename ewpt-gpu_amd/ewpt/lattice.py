@@ -158,6 +158,15 @@ class Lattice:
         P = as_params(p)
         self._check(self._L.ewpt_step(self._h, ctypes.byref(P), int(nsteps)), "ewpt_step")
 
+    def step_diag(self, p, nsteps: int = 1):
+        """nsteps RK4 steps; returns (energy row, Gauss norms) of the state
+        before them (columns 7, 8, 12 of the row are 0: em_energies)."""
+        P = as_params(p)
+        e, g = np.zeros(13), np.zeros(8)
+        self._check(self._L.ewpt_step_diag(self._h, ctypes.byref(P), int(nsteps), _dptr(e), _dptr(g)),
+                    "ewpt_step_diag")
+        return e, g
+
     def step_timed(self, p, nsteps: int):
         """Like step(); returns (mean ms per launch of stages 1..4, total ms)."""
         P = as_params(p)

@@ -70,21 +70,38 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
             _fft_record(lat, p0, theta_w, os.path.join(out_dir, "3d-fft-data-0.ewpt"))
 
         snp = 0
+        pending = None   # snapshot whose energy columns the next step's stage 1 reduces
+        cols = [0, 1, 2, 3, 4, 5, 6, 9, 10, 11]
         t0 = time.perf_counter()
+
+        def report(k, it_k):
+            if verbose:
+                print(f"it={it_k} t*mH={it_k * dt * mH:.3f} E_tot={energies[k, :7].sum():.9e} "
+                      f"B={energies[k, 7]:.6e} ({time.perf_counter() - t0:.1f} s)", flush=True)
+
         for it in range(1, P.nte + 1):
             jex = J_ext(it, dt, mH, P.t_sat, thermal_init=thermal_init)
             gam = damp(it, dt, mH, P.t_sat, P.gamma_inp, thermal_init=thermal_init)
             p = P.step_params(jex=jex, gamma=gam)
-            lat.step(p, 1)
+            if pending is not None:
+                # the energy table of the snapshot state, taken by this step's
+                # stage 1 from the values it stages anyway (ewpt_step_diag)
+                e, _ = lat.step_diag(p, 1)
+                energies[pending[0], cols] = e[cols]
+                report(*pending)
+                pending = None
+            else:
+                lat.step(p, 1)
             if it % dsnaps == 0 and snp < P.nsnaps:
                 snp += 1
-                energies[snp] = lat.energy_row(p, theta_w)
+                energies[snp, [7, 8, 12]] = lat.em_energies(p, theta_w)
                 slices[snp] = np.transpose(lat.y_slice(p, theta_w), (0, 2, 1))
-                if verbose:
-                    print(f"it={it} t*mH={it * dt * mH:.3f} E_tot={energies[snp, :7].sum():.9e} "
-                          f"B={energies[snp, 7]:.6e} ({time.perf_counter() - t0:.1f} s)", flush=True)
+                pending = (snp, it)
             if out_dir and write_fft and it % dsnaps_fft == 0:
                 _fft_record(lat, p, theta_w, os.path.join(out_dir, f"3d-fft-data-{it}.ewpt"))
+        if pending is not None:
+            energies[pending[0], cols] = lat.energies(P.step_params())[cols]
+            report(*pending)
         k1, s1 = lat.spectrum(P.step_params(), theta_w)
         res.update(energies=energies, slices=slices, spectrum=np.stack([np.stack([k0, s0], 1), np.stack([k1, s1], 1)]),
                    gauss=lat.gauss(P.step_params()))

@@ -168,6 +168,20 @@ int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p,
  * stage kernels. */
 int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
 
+/* nsteps RK4 steps as ewpt_step, and the energy table (ewpt_energies
+ * layout: columns 0-6, 9, 11; the EM columns 7, 8, 12 are 0) and the Gauss
+ * norms (ewpt_gauss layout) of the state before the first step.  On the
+ * FP64 path with stage variant 32 (the automatic one where it tiles) the
+ * first step's stage 1 reduces them from the values it stages anyway (the
+ * reductions of main_rk4.jl:6449-6560 and the Gauss residuals of the
+ * k_Gamma / k_Sigma charges, wave-reduced in the stage kernel): no extra
+ * pass over the state.  Otherwise (mixed path, other variants) the
+ * standalone reductions run first.  Equal to ewpt_energies / ewpt_gauss to
+ * rounding (the sums are grouped differently).  A snapshot loop calls it
+ * for the step after the snapshot.  Returns EWPT_ERR_NONFINITE (after
+ * stepping and filling the outputs) if a density is NaN/Inf. */
+int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energies[13], double gauss[8]);
+
 /* Like ewpt_step, and reports mean device time per launch (ms) of each
  * stage kernel, measured with HIP events on the context stream:
  * stage_ms[0..3] = stages 1..4.  *total_ms = event time of the whole run.

@@ -290,3 +290,40 @@ def test_mixed_tracks_fp64_at_512():
             lat.step(P, 1)
             e[prec] = lat.energies(P)
     np.testing.assert_allclose(e[ewpt.EWPT_MIXED][:7], e[ewpt.EWPT_FP64][:7], rtol=1e-8)
+
+
+# ------------------------------------------------ diagnostics fused into stage 1
+
+@pytest.mark.parametrize("precision,shape", [(ewpt.EWPT_FP64, (64, 32, 32)), (ewpt.EWPT_FP64, (32, 16, 24)),
+                                             (ewpt.EWPT_MIXED, (64, 32, 32))])
+def test_step_diag_matches_standalone_and_oracle(precision, shape):
+    """ewpt_step_diag: the energy table and Gauss norms of the state before the
+    step, reduced by the FP64 stage 1 on the way (variant 32), equal the
+    standalone reductions and the oracle to 1e-12; the step itself equals a
+    plain step to a few roundings.  The mixed path takes the standalone
+    reductions."""
+    p = fields.params(gamma=0.1, jex=0.2)
+    s = fields.smooth_state(shape, seed=31, amp=0.4)
+    with ewpt.Lattice(shape, DX, precision=precision) as a, ewpt.Lattice(shape, DX, precision=precision) as b:
+        if precision == ewpt.EWPT_FP64:
+            assert a.active_variant == 332
+        a.set_state(s)
+        b.set_state(s)
+        e_want, g_want = a.energies(p), a.gauss(p)
+        e, g = a.step_diag(p, 2)
+        b.step(p, 2)
+        # the diagnostic stage 1 is compiled separately (other register
+        # allocation, other fused multiply-adds): its step agrees with the plain
+        # one to a few roundings
+        ya, yb = a.get_state(), b.get_state()
+        assert np.abs(ya - yb).max() <= 64 * np.finfo(np.float64).eps * np.abs(yb).max()
+        e2, g2 = a.step_diag(p, 1)                 # the state after two steps
+        e2_want, g2_want = b.energies(p), b.gauss(p)
+    np.testing.assert_allclose(e, e_want, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(g, g_want, rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(e2, e2_want, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(g2, g2_want, rtol=1e-11, atol=1e-15)
+    er, gr = cref.energies(s, p, DX), cref.gauss(s, p, DX)
+    np.testing.assert_allclose(e[[0, 1, 2, 3, 4, 5, 6, 9, 11]], er[[0, 1, 2, 3, 4, 5, 6, 9, 11]], rtol=1e-12)
+    np.testing.assert_allclose(g, gr, rtol=1e-10, atol=1e-14)
+    assert np.all(e[[7, 8, 10, 12]] == 0.0)
