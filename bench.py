@@ -39,7 +39,11 @@ METRIC = "site-updates/s per RK4 step at 512^3, FP64 vs mixed; % of HBM roofline
 HBM_PEAK = 8.0e12
 # Stored-stage-input mask of a stage variant: bit s-2 set = stage s (2..4)
 # reads its input as one array written by stage s-1.
-STORED_MASK = {15: 7, 24: 7, 31: 7, 32: 7, 33: 7}
+STORED_MASK = {15: 7, 24: 7, 31: 7, 32: 7, 33: 7, 34: 7}
+# Variants that rebuild y from the stored inputs and the derivatives instead
+# of reading it: stage 2 reads k1 (20 channels) for the next input's y, and
+# stage 4 reads no y.
+RECON_Y = {34}
 
 
 def stage_bytes_per_site(precision: str, variant: int) -> dict:
@@ -49,18 +53,21 @@ def stage_bytes_per_site(precision: str, variant: int) -> dict:
     k2, writes k3; S4 reads y, k1..k3, writes y'.  A stored input u_s
     replaces the formed reads of stage s by one 36-channel array, and stage
     s-1 then also writes u_s (reading y for it).  Mixed adds the FP32
-    compensation word of y, read and written at stage 4.  Variant 0 or any
-    non-stored variant gives the minimal (formed) figures."""
+    compensation word of y, read and written at stage 4.  A variant that
+    rebuilds y replaces the y read of stage 2 by the 20 k1 channels and drops
+    stage 4's y read.  Variant 0 or any non-stored variant gives the minimal
+    (formed) figures."""
     m = STORED_MASK.get(variant % 100, 0) if variant > 0 else 0
+    recon = variant > 0 and variant % 100 in RECON_Y
     st = {s: s >= 2 and bool((m >> (s - 2)) & 1) for s in (1, 2, 3, 4, 5)}
     formed = {1: 36, 2: 36 + 20, 3: 36 + 16 + 20}
     ch = {}
     for s in (1, 2, 3):
         reads = 36 if st[s] else formed[s]
         if st[s + 1] and st[s]:
-            reads += 36                      # y for the next stage input
+            reads += 20 if recon and s == 2 else 36    # y (or k1) for the next stage input
         ch[s] = reads + 20 + (36 if st[s + 1] else 0)
-    ch[4] = 36 + 60 + (36 if st[4] else 0) + 36
+    ch[4] = 36 + 60 + (36 if st[4] and not recon else 0) + 36
     word = 8 if precision == "fp64" else 4
     if precision != "fp64":
         ch[4] += 36 + 36
@@ -244,40 +251,20 @@ def corrector_bench(a) -> int:
 
     import ewpt
 
+    from ewpt.corrector import train_thermal
+
     dx, factor = 0.25, 4
     n_train, n_eval, fine_steps = a.corr_train_n, a.corr_eval_n, a.corr_fine_steps
-    par = ewpt.Parameters(N=n_train, dx=dx, gy=0.35)
-    p = par.step_params()
-    rng = np.random.default_rng(7)
     t0 = time.perf_counter()
-    src, co, fi = (ewpt.Lattice(n_train, dx) for _ in range(3))
     net = ewpt.Corrector(seed=3)
     net.input_mode = 1            # lattice Laplacian of the coarse state
-    try:
-        cs, rs = [], []
-        for k in range(6):
-            src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 900 + k, "config")
-            src.step(p, int(k * fine_steps / 12))
-            ewpt.make_pair(src, co, fi, p, factor)
-            c, f = co.get_state(), fi.get_state()
-            cs.append(net.features(c).reshape(36, -1).std(1))
-            rs.append((f - c).reshape(36, -1).std(1))
-        net.set_norm(np.maximum(np.mean(cs, 0), 1e-12), np.maximum(np.mean(rs, 0), 1e-30))
-        losses = []
-        for it in range(a.corr_iters):
-            if it % 100 == 0:   # a new trajectory: fresh seed and temperature, random start time
-                src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 1000 + it, "config")
-                src.step(p, int(rng.integers(0, fine_steps // 2)))
-            ewpt.make_pair(src, co, fi, p, factor)
-            losses.append(net.train_step(co, fi, a.corr_lr))
-            src.step(p, int(rng.integers(1, 5)))
-            if (it + 1) % 250 == 0:
-                print(f"bench: corrector training {it + 1}/{a.corr_iters}, relative loss "
-                      f"{np.mean(losses[-100:]):.3f} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-        train_s = time.perf_counter() - t0
-    finally:
-        for x in (src, co, fi):
-            x.close()
+
+    def log(it, loss):
+        print(f"bench: corrector training {it}/{a.corr_iters}, relative loss {loss:.3f} "
+              f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    losses = train_thermal(net, n_train, a.corr_iters, a.corr_lr, factor, fine_steps, dx, log=log)
+    train_s = time.perf_counter() - t0
     print(f"bench: corrector trained {a.corr_iters} iterations on {n_train}^3 in {train_s:.1f} s, "
           f"relative loss {np.mean(losses[:50]):.3f} -> {np.mean(losses[-200:]):.3f}", file=sys.stderr, flush=True)
 

@@ -208,11 +208,11 @@ int push_view(ewpt_ctx *c)
 }
 
 // variant actually launched: AUTO takes the first brick variant that tiles
-// the lattice from the measured ranking (512^3 on MI355X, FP64: 332 141.0,
-// 324 170.3, 320 181.2, 311 184.4, 310 187.0 ms/step; mixed: 333 78.9,
-// 331 82.7, 315 89.4, 311 96.3, 211 103.2 ms/step), else the gather kernel.
-// The stored-stage-input variants (332, 333, ...) hold one more state-sized
-// buffer; without room for it (or on a slab) AUTO skips them.
+// the lattice from the measured ranking (512^3 on MI355X, FP64: 334 130.7,
+// 332 140.6, 324 170.3, 320 181.2, 311 184.4, 310 187.0 ms/step; mixed:
+// 333 78.0, 331 82.7, 315 89.4, 311 96.3, 211 103.2 ms/step), else the
+// gather kernel.  The stored-stage-input variants (334, 333, ...) hold one
+// more state-sized buffer; without room for it (or on a slab) AUTO skips them.
 int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
@@ -221,7 +221,7 @@ int resolved_variant(const ewpt_ctx *c)
             if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported_f32(v, c->lat)) return v;
         return -1;
     }
-    for (int v : {332, 324, 325, 320, 311, 310, 210, 211, 2})
+    for (int v : {334, 332, 324, 325, 320, 311, 310, 210, 211, 2})
         if ((c->stored_ok || !brick_stored_inputs(v)) && brick_supported(v, c->lat)) return v;
     return VARIANT_GATHER;
 }
@@ -698,10 +698,34 @@ int ewpt_init_thermal_gauss(ewpt_ctx *ctx, double T, double meff_sq, unsigned lo
     return EWPT_OK;
 }
 
+namespace {
+
+// One rk4_step! evaluation on device buffers: the stage-1 brick kernel of a
+// variant that forms its input on load (it reads only `in`) writes k_Gamma,
+// k_Sigma and kt into channels 16..35 of out; channels 0..15 are the momenta
+// (k_f = pi), copied.  Lattices no brick tiles take the gather kernel.
+hipError_t launch_rhs(const double *in, double *out, const ewpt_lattice &L, const Phys &P, hipStream_t st)
+{
+    for (int v : {325, 311, 310, 2}) {
+        if (!brick_supported(v, L) || brick_stored_inputs(v)) continue;
+        const long n3 = (long)L.nx * L.ny * L.nz;
+        StageBufs<double> b{};
+        b.y = in;
+        b.kout = out + 16 * n3;
+        b.n3 = n3;
+        hipError_t e = launch_stage_brick(v, 1, b, L, P, 0.0, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(out, in + 20 * n3, 16 * n3 * sizeof(double), hipMemcpyDeviceToDevice, st);
+        return e;
+    }
+    return launch_rhs_gather(in, out, L, P, st);
+}
+
+}  // namespace
+
 int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p, const double *din, double *dout, void *stream)
 {
     if (!lattice_ok(lat) || !params_ok(p) || !din || !dout || din == dout) return EWPT_ERR_INVAL;
-    const hipError_t e = launch_rhs_gather(din, dout, *lat, phys_of(*p), static_cast<hipStream_t>(stream));
+    const hipError_t e = launch_rhs(din, dout, *lat, phys_of(*p), static_cast<hipStream_t>(stream));
     return e == hipSuccess ? EWPT_OK : EWPT_ERR_HIP;
 }
 
@@ -715,7 +739,7 @@ int ewpt_rhs(ewpt_ctx *ctx, const ewpt_params *p, const double *hin, double *hou
     if (!ctx->rhs_in && (rc = dev_alloc(ctx, &ctx->rhs_in, n, "alloc rhs in"))) return rc;
     if (!ctx->rhs_out && (rc = dev_alloc(ctx, &ctx->rhs_out, n, "alloc rhs out"))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(ctx->rhs_in, hin, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "rhs in copy");
-    HIPCHK(ctx, launch_rhs_gather(ctx->rhs_in, ctx->rhs_out, ctx->lat, phys_of(*p), ctx->stream), "rhs launch");
+    HIPCHK(ctx, launch_rhs(ctx->rhs_in, ctx->rhs_out, ctx->lat, phys_of(*p), ctx->stream), "rhs launch");
     HIPCHK(ctx, hipMemcpyAsync(hout, ctx->rhs_out, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "rhs out copy");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "rhs sync");
     return EWPT_OK;
@@ -1237,7 +1261,7 @@ void drop_plan(ewpt_batch *b)
 std::vector<int> batch_candidates(bool f32)
 {
     std::vector<int> v = f32 ? std::vector<int>{333, 331, 315, 311, 211, 310, 210, 11, 10}
-                             : std::vector<int>{332, 311, 310, 211, 210, 11, 10, 2};
+                             : std::vector<int>{334, 332, 311, 310, 211, 210, 11, 10, 2};
     const size_t n = v.size();
     for (size_t i = 0; i < n; i++) v.push_back(1000 + v[i]);   // persistent forms
     return v;

@@ -88,7 +88,7 @@ bool tile_of(int v, bool f32, int &tx, int &ty, int &tz)
     }
     switch (v) {
     case 2: tx = 8; ty = 8; tz = 8; return true;
-    case 32: tx = 16; ty = 4; tz = 4; return true;
+    case 32: case 34: tx = 16; ty = 4; tz = 4; return true;
     default: return false;
     }
 }
@@ -204,6 +204,7 @@ auto with_kern(int v, F &&f)
     if constexpr (std::is_same_v<T, double>) {
         if (v == 2) return f(Kern<8, 8, 8, 1, T>{});
         if (v == 32) return f(Kern<16, 4, 4, 211704432, T>{});
+        if (v == 34) return f(Kern<16, 4, 4, 311704432, T>{});
     }
     return f(Kern<16, 8, 4, 4432, T>{});
 }

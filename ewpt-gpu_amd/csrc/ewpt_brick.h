@@ -302,7 +302,7 @@ __host__ __device__ constexpr bool park_centres(int PF, int S)
 // before the first pass (2: fused_fields as well).
 __host__ __device__ constexpr bool spread_update(int PF)
 {
-    return (PF / 100000000) % 10 == 1 || (PF / 100000000) % 10 == 2;
+    return (PF / 100000000) % 10 >= 1 && (PF / 100000000) % 10 <= 3;
 }
 __host__ __device__ constexpr bool stores_next(int PF, int S) { return S < 4 && stored_in(PF, S + 1); }
 // PF / 1000000000 = 1 (mixed): stage 4 parks only the increment in LDS and
@@ -314,7 +314,16 @@ __host__ __device__ constexpr bool lean_stash(int PF) { return (PF / 1000000000)
 // sum of channel ch + 20, its momentum) from the same loads of the momentum
 // y and k's: a separate field loop at the start loaded those 64 channels a
 // second time, after L2 had evicted them.
-__host__ __device__ constexpr bool fused_fields(int PF) { return (PF / 100000000) % 10 == 2; }
+__host__ __device__ constexpr bool fused_fields(int PF) { return (PF / 100000000) % 10 == 2 || (PF / 100000000) % 10 == 3; }
+// PF / 100000000 = 3 (FP64, stored stage inputs): as 2, and y is never read
+// after stage 1: it is reconstructed from the stage input and the stored
+// derivatives.  Stage 4: y = u4 - k3 dt (momenta, Gamma, Sigma) and
+// y_f = u4_f - k3_f dt with k3_f = y_pi + kt2 dt/2 (fields), from the u4 centres
+// and the k loads of the update; its partial sums run in the order the
+// passes stage the centres (E, Gamma/Sigma, then pi).  Stage 2 forms u3 from
+// u2 and k1 (20 channels) instead of y (36).  Each reconstruction rounds once
+// more than the reference's y + c k (about one unit in the last place).
+__host__ __device__ constexpr bool recon_y(int PF) { return (PF / 100000000) % 10 == 3; }
 __host__ __device__ constexpr int waves_per_simd(bool f32, int PF, int S) { return f32 && (S < 4 || lean_stash(PF)) ? 4 : 2; }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
@@ -853,14 +862,23 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     // so their y is held until the end (loaded in the last YP/2 passes).
     constexpr bool YE = y_early(PF) && S > 1 && stores_next(PF, S) && !paired_of(PF, S);
     constexpr bool YF = YE && early_fields(PF);
+    // recon_y, stage 2: k1 (20 channels) instead of y (36) in the last passes;
+    // u3 = y + k2 dt/2 with y = u2 - k1 dt/2 (momenta, Gamma, Sigma) and, for a
+    // field, y_f = u2_f - y_pi dt/2, y_pi = u2_pi - kt1 dt/2
+    constexpr bool YK = YE && recon_y(PF) && S == 2;
     constexpr int YP = 8;
-    R ypre[YE ? NCH : 1];
+    R ypre[YE && !YK ? NCH : 1];
+    R kpre[YK ? NK : 1];
     if constexpr (!paired_of(PF, S) && !SPREAD) {
         store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
         lds_barrier();
         auto body = [&](auto pc) {
             constexpr int pp = decltype(pc)::value;
-            if constexpr (YE && !YF && pp >= NPASS - YP) {
+            if constexpr (YK && pp >= NPASS - YP) {
+                constexpr int q = pp - (NPASS - YP);
+#pragma unroll
+                for (int m = q * NK / YP; m < (q + 1) * NK / YP; m++) kpre[m] = ldo(K(b.k1, m), qb);
+            } else if constexpr (YE && !YF && pp >= NPASS - YP) {
                 constexpr int q = pp - (NPASS - YP);
 #pragma unroll
                 for (int ch = q * NCH / YP; ch < (q + 1) * NCH / YP; ch++) ypre[ch] = ldo(Y(ch), qb);
@@ -922,6 +940,32 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                     two_sum(ldo(Y(ch), qb), ldo(b.ylo + (long)ch * b.n3, qb) + dyf, nh, nl);
                     stn(b.ynew + (long)ch * b.n3, qb, nh);
                     stn(b.ylo_new + (long)ch * b.n3, qb, nl);
+                }
+            } else if constexpr (pp < NK && recon_y(PF)) {
+                // the partial sum of channel 16+m, m in the order the passes set
+                // the centres of the stage input (E, Gamma/Sigma, pi), with y
+                // reconstructed from that centre: u4 = y + k3 dt
+                constexpr int m = pp < 16 ? pass_channel(pp) - 16 : 4 + (pp - 16);
+                static_assert(stored_in(PF, 4), "y is reconstructed from the stored stage input");
+                const R k1v = ldo(K(b.k1, m), qb), k2v = ldo(K(b.k2, m), qb), k3v = ldo(K(b.k3, m), qb);
+                consume(pc, lds[pp & 1]);
+                pin_all(pc);
+                const R yv = c[16 + m] - k3v * dt;
+                R n = k1v * d6 + yv;
+                n = k2v * d3 + n;
+                nstash[m * B::NT + t] = k3v * d3 + n;
+                if constexpr (m >= 4) {
+                    // field channel m - 4, whose momentum is channel 16 + m:
+                    // u4_f = y_f + k3_f dt
+                    constexpr int ch = m - 4;
+                    const R piv = yv;
+                    const R k2 = piv + k1v * half_dt;
+                    const R k3 = piv + k2v * half_dt;
+                    const R k4 = piv + k3v * dt;
+                    R nf = piv * d6 + (c[ch] - k3 * dt);
+                    nf = k2 * d3 + nf;
+                    nf = k3 * d3 + nf;
+                    stn(b.ynew + (long)ch * b.n3, qb, k4 * d6 + nf);
                 }
             } else if constexpr (pp < NK) {
                 // the partial sum of channel 16+pp: its loads are in flight while
@@ -1009,6 +1053,8 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 R yv;
                 if constexpr (PARK) yv = ch >= 16 ? park[(ch - 16) * B::NT + t] : c[ch];
                 else if constexpr (S == 1) yv = c[ch];
+                else if constexpr (YK) yv = ch < 16 ? c[ch] - (c[ch + 20] - kpre[ch + 4] * half_dt) * half_dt
+                                                    : c[ch] - kpre[ch - 16] * half_dt;
                 else if constexpr (YE) yv = ypre[ch];
                 else yv = R(ldo(Y(ch), qb));
                 const R kv = ch < 16 ? c[ch + 20] : o[ch - 16];

@@ -15,6 +15,9 @@
 //       stages 2-3, stage 4's partial-sum inputs loaded one channel per pass
 //       and the field channels of y' updated from those same loads; the same
 //       arithmetic in the same order as 24 (bitwise equal results)
+//   34: 32 with y reconstructed from the stored stage inputs and derivatives
+//       (stage 4 reads no y, stage 2 reads k1 instead of y): fewer bytes, one
+//       more rounding per reconstructed value
 // All use the XCD-aware block order; + 100 * o selects brick enumeration
 // order o (brick_coords).
 #include "ewpt_internal.h"
@@ -32,6 +35,7 @@ EWPT_DECLARE_BRICK(20)
 EWPT_DECLARE_BRICK(24)
 EWPT_DECLARE_BRICK(25)
 EWPT_DECLARE_BRICK(32)
+EWPT_DECLARE_BRICK(34)
 #undef EWPT_DECLARE_BRICK
 hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                                  int order, double *diag, hipStream_t st);
@@ -49,7 +53,7 @@ constexpr BrickEntry kBricks[] = {
     {2, brick_fits_v2, launch_brick_v2},     {10, brick_fits_v10, launch_brick_v10},
     {11, brick_fits_v11, launch_brick_v11},  {20, brick_fits_v20, launch_brick_v20},
     {24, brick_fits_v24, launch_brick_v24},  {25, brick_fits_v25, launch_brick_v25},
-    {32, brick_fits_v32, launch_brick_v32},
+    {32, brick_fits_v32, launch_brick_v32},  {34, brick_fits_v34, launch_brick_v34},
 };
 
 const BrickEntry *find_brick(int variant)
@@ -78,10 +82,15 @@ bool brick_supported(int variant, const ewpt_lattice &L)
 
 long diag_rows(const ewpt_lattice &L)
 {
-    return (long)(L.nx / 16) * (L.ny / 4) * (L.nz / 4);   // the 16x4x4 bricks of variant 32
+    return (long)(L.nx / 16) * (L.ny / 4) * (L.nz / 4);   // the 16x4x4 bricks of variants 32, 34
 }
 
-bool brick_has_diag(int variant) { return variant >= 0 && variant % 100 == 32 && brick_exists(variant); }
+// Variants 32 and 34 share their stage-1 kernel (34 differs in stages 2 and
+// 4 only), so both take variant 32's stage 1 with the fused reductions.
+bool brick_has_diag(int variant)
+{
+    return variant >= 0 && (variant % 100 == 32 || variant % 100 == 34) && brick_exists(variant);
+}
 
 hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                               double *diag, hipStream_t st)

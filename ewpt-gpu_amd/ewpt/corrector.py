@@ -144,3 +144,42 @@ def make_pair(src: Lattice, coarse: Lattice, fine: Lattice, p, factor: int = 4):
     coarse.step(pc, 1)
     fine.copy_from(src)
     fine.step(P, factor)
+
+
+def train_thermal(net: "Corrector", n: int, iters: int, lr: float = 3e-4, factor: int = 4, horizon: int = 2000,
+                  dx: float = 0.25, gy: float = 0.35, seed: int = 7, log=None) -> list:
+    """Train net on n^3 thermal trajectories (T uniform in [0.3, 0.7]): each
+    pair is one factor*dt RK4 step against factor dt steps from the same
+    state.  The channel scales come from six states spread over the first
+    `horizon` dt steps; a new trajectory (fresh seed and temperature, random
+    start in the first horizon/2 steps) begins every 100 pairs, and the
+    source advances 1-4 steps between pairs.  Returns the relative losses."""
+    from .parameters import Parameters
+    par = Parameters(N=n, dx=dx, gy=gy)
+    p = par.step_params()
+    rng = np.random.default_rng(seed)
+    src, co, fi = (Lattice(n, dx) for _ in range(3))
+    try:
+        cs, rs = [], []
+        for k in range(6):
+            src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 900 + k, "config")
+            src.step(p, int(k * horizon / 12))
+            make_pair(src, co, fi, p, factor)
+            c, f = co.get_state(), fi.get_state()
+            cs.append(net.features(c).reshape(NCH, -1).std(1))
+            rs.append((f - c).reshape(NCH, -1).std(1))
+        net.set_norm(np.maximum(np.mean(cs, 0), 1e-12), np.maximum(np.mean(rs, 0), 1e-30))
+        losses = []
+        for it in range(iters):
+            if it % 100 == 0:
+                src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 1000 + it, "config")
+                src.step(p, int(rng.integers(0, horizon // 2)))
+            make_pair(src, co, fi, p, factor)
+            losses.append(net.train_step(co, fi, lr))
+            src.step(p, int(rng.integers(1, 5)))
+            if log and (it + 1) % 250 == 0:
+                log(it + 1, float(np.mean(losses[-100:])))
+    finally:
+        for x in (src, co, fi):
+            x.close()
+    return losses

@@ -158,7 +158,9 @@ int ewpt_init_thermal_gauss(ewpt_ctx *ctx, double T, double meff_sq, unsigned lo
 int ewpt_rhs(ewpt_ctx *ctx, const ewpt_params *p, const double *host_in, double *host_out);
 
 /* Same on device buffers (FP64, 36*nx*ny*nz each, distinct), enqueued on
- * `stream` (a hipStream_t, NULL = default stream); asynchronous. */
+ * `stream` (a hipStream_t, NULL = default stream); asynchronous.  Both run
+ * the LDS-brick stage-1 kernel (16x4x4, 32x4x4, 16x8x4 or 8x8x8 bricks,
+ * whichever tiles the lattice; else the one-site-per-thread gather kernel). */
 int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p,
                     const double *dev_in, double *dev_out, void *stream);
 

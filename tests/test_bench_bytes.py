@@ -33,6 +33,15 @@ def test_stored_stage_inputs():
     assert mixed[4] == (36 + 36 + 60 + 36 + 36 + 36) * 4
 
 
+def test_rebuilt_y():
+    # y rebuilt from u2 and k1 at stage 2, from u4 and the k's at stage 4:
+    # S2 u2 + k1 + k2 + u3; S4 u4 + k1..k3 + y'
+    want = {1: (36 + 20 + 36) * 8, 2: (36 + 20 + 20 + 36) * 8, 3: (36 + 36 + 20 + 36) * 8,
+            4: (36 + 60 + 36) * 8}
+    assert bench.stage_bytes_per_site("fp64", 334) == want
+    assert sum(want.values()) == 3712
+
+
 def test_measured_traffic_only_at_512_with_source():
     assert bench.measured_traffic("fp64", 332, 4, 256) == (None, None)
     assert bench.measured_traffic("fp64", 999, 4, 512) == (None, None)

@@ -36,7 +36,8 @@ def _params(k):
 
 @pytest.mark.parametrize("precision,variant", [(ewpt.EWPT_MIXED, 210), (ewpt.EWPT_MIXED, 1010),
                                                (ewpt.EWPT_FP64, 210), (ewpt.EWPT_FP64, 1002),
-                                               (ewpt.EWPT_FP64, 332), (ewpt.EWPT_FP64, 1332)])
+                                               (ewpt.EWPT_FP64, 332), (ewpt.EWPT_FP64, 1332),
+                                               (ewpt.EWPT_FP64, 334), (ewpt.EWPT_FP64, 1334)])
 def test_batch_bitwise_equals_single(precision, variant):
     shapes = [(32, 32, 32), (48, 48, 48), (64, 32, 16), (32, 48, 64)]
     print(f"batch precision={precision} variant={variant}", flush=True)

@@ -21,36 +21,15 @@ DX = 0.25
 
 
 def test_corrector_config_128_train_256_eval():
+    from ewpt.corrector import train_thermal
+
     n_train, n_eval, factor, pairs, fine_steps = 128, 256, 4, 800, 400
-    par = ewpt.Parameters(N=n_train, dx=DX, gy=0.35)
-    p = par.step_params()
-    rng = np.random.default_rng(11)
-    src, co, fi = (ewpt.Lattice(n_train, DX) for _ in range(3))
-    net = ewpt.Corrector(seed=5)
+    net = ewpt.Corrector(seed=3)
     net.input_mode = 1
     t0 = time.perf_counter()
-    try:
-        cs, rs = [], []
-        for k in range(4):
-            src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 300 + k, "config")
-            src.step(p, 60 * k)
-            ewpt.make_pair(src, co, fi, p, factor)
-            c, f = co.get_state(), fi.get_state()
-            cs.append(net.features(c).reshape(36, -1).std(1))
-            rs.append((f - c).reshape(36, -1).std(1))
-        net.set_norm(np.maximum(np.mean(cs, 0), 1e-12), np.maximum(np.mean(rs, 0), 1e-30))
-        losses = []
-        for it in range(pairs):
-            if it % 100 == 0:
-                src.init_thermal(float(rng.uniform(0.3, 0.7)), par.meff_sq, 600 + it, "config")
-                src.step(p, int(rng.integers(0, fine_steps // 2)))
-                print(f"corrector training {it}/{pairs} ({time.perf_counter() - t0:.0f} s)", flush=True)
-            ewpt.make_pair(src, co, fi, p, factor)
-            losses.append(net.train_step(co, fi, 3e-4))
-            src.step(p, int(rng.integers(1, 5)))
-    finally:
-        for x in (src, co, fi):
-            x.close()
+    losses = train_thermal(net, n_train, pairs, 3e-4, factor, 2000, DX,
+                           log=lambda it, l: print(f"corrector training {it}/{pairs}: relative loss {l:.3f} "
+                                                   f"({time.perf_counter() - t0:.0f} s)", flush=True))
     print(f"relative loss {np.mean(losses[:50]):.3f} -> {np.mean(losses[-100:]):.3f}", flush=True)
     assert np.mean(losses[-100:]) < np.mean(losses[:50])
 
