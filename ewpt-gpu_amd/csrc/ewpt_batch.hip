@@ -177,7 +177,7 @@ struct BatchPlan {
 bool batch_variant_fits(int variant, bool f32, const ewpt_lattice &L)
 {
     int tx, ty, tz;
-    if (variant < 0 || variant >= 2000 || (variant % 1000) / 100 > 4 || !tile_of(variant % 100, f32, tx, ty, tz))
+    if (variant < 0 || variant >= 2000 || (variant % 1000) / 100 > 9 || !tile_of(variant % 100, f32, tx, ty, tz))
         return false;
     return L.nx % tx == 0 && L.ny % ty == 0 && L.nz % tz == 0 && (long)L.nx * L.ny * L.nz * 8 < (1L << 31);
 }

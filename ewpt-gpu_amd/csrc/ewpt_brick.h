@@ -88,11 +88,20 @@ __device__ __forceinline__ int wrapc(int v, int n) { return v < 0 ? v + n : (v >
 //   3: cubes of 2x4x4 bricks (x,y,z), 32 consecutive indices per cube: the
 //      workgroups one XCD runs at the same time share most of their halos
 //   4: cubes of 4x4x4 bricks (64 per cube), for two workgroups per CU
-// Orders 3, 4 fall back to 0 where the brick counts do not divide.
+//   5: 2x8x4, 6: 2x4x8, 7: 1x8x8, 8: 2x8x8, 9: 1x4x4 bricks per cube
+// Orders 3-9 fall back to 0 where the brick counts do not divide.
+__host__ __device__ constexpr int cube_bricks(int order, int axis)
+{
+    // x, y, z extents packed one hex digit each, orders 3..9
+    constexpr int kCube[7] = {0x244, 0x444, 0x284, 0x248, 0x188, 0x288, 0x144};
+    return (kCube[order - 3] >> (4 * (2 - axis))) & 15;
+}
+
 __device__ __forceinline__ void brick_coords(const BrickGeo &g, int i, int &bx, int &by, int &bz)
 {
     if (g.order >= 3) {
-        const int cx = g.order == 3 ? 2 : 4, cy = 4, cz = 4, cn = cx * cy * cz;
+        const int cx = cube_bricks(g.order, 0), cy = cube_bricks(g.order, 1), cz = cube_bricks(g.order, 2);
+        const int cn = cx * cy * cz;
         if (g.nbx % cx == 0 && g.nby % cy == 0 && g.nbz % cz == 0) {
             const int cube = i / cn, in = i % cn;
             const int ncx = g.nbx / cx, ncy = g.nby / cy;

@@ -58,7 +58,7 @@ constexpr BrickEntry kBricks[] = {
 
 const BrickEntry *find_brick(int variant)
 {
-    if (variant < 0 || variant / 100 > 4) return nullptr;
+    if (variant < 0 || variant / 100 > 9) return nullptr;
     for (const BrickEntry &e : kBricks)
         if (e.v == variant % 100) return &e;
     return nullptr;

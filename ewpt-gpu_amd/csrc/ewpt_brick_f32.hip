@@ -22,7 +22,7 @@ namespace ewpt {
 
 bool brick_exists_f32(int variant)
 {
-    if (variant < 0 || variant / 100 > 4) return false;
+    if (variant < 0 || variant / 100 > 9) return false;
     switch (variant % 100) {
     case 10: case 11: case 15: case 24: case 25: case 31: case 33: return true;
     default: return false;

@@ -339,8 +339,8 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *       first selected).  Prefetch depth 4/4/3/2 by stage unless stated.
  *   9 = gather kernel (FP64; one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
- * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3 = cubes of 2x4x4
- * bricks; 4 = cubes of 4x4x4 bricks), e.g. 311.
+ * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3-9 = cubes of
+ * 2x4x4, 4x4x4, 2x8x4, 2x4x8, 1x8x8, 2x8x8, 1x4x4 bricks), e.g. 311.
  * Setting a variant the precision does not have returns EWPT_ERR_INVAL; one
  * that does not tile the lattice returns EWPT_ERR_UNSUPPORTED. */
 int ewpt_set_variant(ewpt_ctx *ctx, int variant);

@@ -51,8 +51,8 @@ def test_measured_traffic_only_at_512_with_source():
 
 
 def test_default_variants_have_measured_traffic():
-    """The automatic variants at 512^3 (FP64 332, mixed 333) carry counter
+    """The automatic variants at 512^3 (FP64 334, mixed 333) carry counter
     figures for all four stages, so the default line's traffic is never null."""
-    for prec, v in (("fp64", 332), ("mixed", 333)):
+    for prec, v in (("fp64", 334), ("mixed", 333)):
         for s in (1, 2, 3, 4):
             assert (prec, v, s) in bench.MEASURED_TRAFFIC

@@ -129,7 +129,7 @@ def test_stage_variants_match_oracle(variant, beta):
     assert err.max() < 1e-12, (variant, err.argmax(), err.max())
 
 
-@pytest.mark.parametrize("variant", [310, 311, 410, 411, 332, 432, 334])
+@pytest.mark.parametrize("variant", [310, 311, 410, 411, 332, 432, 334, 534, 634, 734, 834, 934])
 def test_cube_orders_match_oracle(variant):
     # 64x32x32 holds whole cubes of bricks for both brick shapes
     shape = (64, 32, 32)
@@ -228,7 +228,7 @@ def test_mixed_state_roundtrip():
     assert np.abs(y - s).max() <= 1e-13 * np.abs(s).max()
 
 
-@pytest.mark.parametrize("variant", [0, 10, 11, 15, 24, 25, 31, 33, 210, 311, 315, 324, 325, 331, 333, 410])
+@pytest.mark.parametrize("variant", [0, 10, 11, 15, 24, 25, 31, 33, 210, 311, 315, 324, 325, 331, 333, 410, 533, 933])
 def test_mixed_steps_track_fp64(variant):
     """FP32 storage/stencils with the compensated update stay within the
     stated error of the FP64 oracle: 1e-5 relative L2 per field channel
