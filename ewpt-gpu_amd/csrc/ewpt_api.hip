@@ -522,6 +522,43 @@ int ewpt_get_state_device(ewpt_ctx *ctx, double *dev_state)
     return EWPT_OK;
 }
 
+int ewpt_get_state_pair(ewpt_ctx *ctx, float *hi, float *lo)
+{
+    if (!ctx || !hi || !lo) return EWPT_ERR_INVAL;
+    if (!mixed(ctx)) return fail(ctx, EWPT_ERR_INVAL, "state pair of an FP64 context");
+    if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "get_state_pair before set_state");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const long off = ctx->zg * ctx->plane;
+    HIPCHK(ctx, copy_channels(hi, sites(ctx), 0, ctx->fhi, ctx->n3, off, NCH, sites(ctx), sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream),
+           "get_state_pair copy");
+    HIPCHK(ctx, copy_channels(lo, sites(ctx), 0, ctx->flo, ctx->n3, off, NCH, sites(ctx), sizeof(float),
+                              hipMemcpyDeviceToHost, ctx->stream),
+           "get_state_pair copy");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "get_state_pair sync");
+    return EWPT_OK;
+}
+
+int ewpt_set_state_pair(ewpt_ctx *ctx, const float *hi, const float *lo)
+{
+    if (!ctx || !hi || !lo) return EWPT_ERR_INVAL;
+    if (!mixed(ctx)) return fail(ctx, EWPT_ERR_INVAL, "state pair of an FP64 context");
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    const long off = ctx->zg * ctx->plane;
+    HIPCHK(ctx, copy_channels(ctx->fhi, ctx->n3, off, hi, sites(ctx), 0, NCH, sites(ctx), sizeof(float),
+                              hipMemcpyHostToDevice, ctx->stream),
+           "set_state_pair copy");
+    HIPCHK(ctx, copy_channels(ctx->flo, ctx->n3, off, lo, sites(ctx), 0, NCH, sites(ctx), sizeof(float),
+                              hipMemcpyHostToDevice, ctx->stream),
+           "set_state_pair copy");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "set_state_pair sync");
+    ctx->view_valid = false;   // the FP64 view is joined from the pair when next read
+    ctx->has_state = true;
+    return EWPT_OK;
+}
+
 int ewpt_copy_probe(const double *dev_src, double *dev_dst, long n, int reps, void *stream, double *ms)
 {
     if ((!dev_src) != (!dev_dst) || n < 1 || reps < 1 || !ms) return EWPT_ERR_INVAL;

@@ -3,6 +3,7 @@
 Mirrors the driver surface of Teerthal/EWPT-GPU (gpu-kernels-rk4/main_rk4.jl
 run(), parameters.jl) over the C ABI of libewpt (include/ewpt.h).
 """
+from . import checkpoint
 from ._lib import (EWPT_FP64, EWPT_MIXED, EwptError, EwptLattice, EwptParams, NCH, NK, load)
 from .batch import Batch
 from .corrector import Corrector, make_pair
@@ -12,4 +13,4 @@ from .parameters import Parameters, J_ext, damp
 
 __all__ = ["EWPT_FP64", "EWPT_MIXED", "EwptError", "EwptLattice", "EwptParams", "NCH", "NK", "load",
            "Batch", "Corrector", "make_pair", "Lattice", "as_params", "slab_step", "slab_step_timed", "bubble_table",
-           "gen_phi", "Parameters", "J_ext", "damp"]
+           "gen_phi", "Parameters", "J_ext", "damp", "checkpoint"]

@@ -41,7 +41,8 @@ EXPORTS = (
     "ewpt_net_set_input", "ewpt_copy_probe", "ewpt_create_slab", "ewpt_slab_halo_bytes", "ewpt_slab_buffers",
     "ewpt_slab_pack", "ewpt_slab_unpack", "ewpt_stage", "ewpt_get_stream", "ewpt_sync", "ewpt_slab_step",
     "ewpt_slab_step_timed", "ewpt_set_state_device", "ewpt_get_state_device",
-    "ewpt_energies_device", "ewpt_init_thermal_gauss", "ewpt_step_diag",
+    "ewpt_energies_device", "ewpt_init_thermal_gauss", "ewpt_step_diag", "ewpt_get_state_pair",
+    "ewpt_set_state_pair",
 )
 
 
@@ -86,6 +87,8 @@ def load() -> ctypes.CDLL:
         "ewpt_set_state_device": [vp, vp],
         "ewpt_energies_device": [lp, pp, vp, vp, vp],
         "ewpt_get_state_device": [vp, vp],
+        "ewpt_get_state_pair": [vp, fp, fp],
+        "ewpt_set_state_pair": [vp, fp, fp],
         "ewpt_rhs": [vp, pp, dp, dp],
         "ewpt_rhs_device": [lp, pp, vp, vp, vp],
         "ewpt_step": [vp, pp, ip],

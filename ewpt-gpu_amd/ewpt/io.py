@@ -6,8 +6,8 @@ Container layout (all integers little-endian)::
 
     file    := magic record*            magic = b"EWPTDAT1"
     record  := u16 name_len, name (UTF-8, name_len bytes),
-               4-byte dtype code (b"f8\\0\\0" float64, b"i8\\0\\0" int64,
-                                  b"c16\\0" complex128),
+               4-byte dtype code (b"f8\\0\\0" float64, b"f4\\0\\0" float32,
+                                  b"i8\\0\\0" int64, b"c16\\0" complex128),
                u32 ndim, u64 dims[ndim],
                u64 nbytes, zero padding to the next multiple of 64 bytes
                (offset from the file start), nbytes of data
@@ -40,7 +40,8 @@ import struct
 import numpy as np
 
 MAGIC = b"EWPTDAT1"
-_CODES = {np.dtype(np.float64): b"f8\0\0", np.dtype(np.int64): b"i8\0\0", np.dtype(np.complex128): b"c16\0"}
+_CODES = {np.dtype(np.float64): b"f8\0\0", np.dtype(np.float32): b"f4\0\0", np.dtype(np.int64): b"i8\0\0",
+          np.dtype(np.complex128): b"c16\0"}
 _TYPES = {v: k for k, v in _CODES.items()}
 _ALIGN = 64
 
@@ -50,7 +51,7 @@ def _as_dataset(value) -> np.ndarray:
     if a.dtype.kind in "biu":
         a = a.astype(np.int64)
     elif a.dtype.kind == "f":
-        a = a.astype(np.float64)
+        a = a if a.dtype == np.float32 else a.astype(np.float64)
     elif a.dtype.kind == "c":
         a = a.astype(np.complex128)
     else:

@@ -134,6 +134,23 @@ class Lattice:
         self._check(self._L.ewpt_get_state(self._h, _dptr(out)), "ewpt_get_state")
         return out
 
+    def get_state_pair(self) -> tuple[np.ndarray, np.ndarray]:
+        """Mixed path: the FP32 pair (hi, lo) the state is evolved in."""
+        hi, lo = np.empty(self.shape, np.float32), np.empty(self.shape, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.ewpt_get_state_pair(self._h, hi.ctypes.data_as(fp), lo.ctypes.data_as(fp)),
+                    "ewpt_get_state_pair")
+        return hi, lo
+
+    def set_state_pair(self, hi: np.ndarray, lo: np.ndarray):
+        """Mixed path: set the state as its FP32 pair (state = hi + lo)."""
+        a, b = (np.ascontiguousarray(x, dtype=np.float32) for x in (hi, lo))
+        if a.shape != self.shape or b.shape != self.shape:
+            raise ValueError(f"state pair shapes {a.shape}, {b.shape} != {self.shape}")
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.ewpt_set_state_pair(self._h, a.ctypes.data_as(fp), b.ctypes.data_as(fp)),
+                    "ewpt_set_state_pair")
+
     def set_state_device(self, ptr: int):
         """Copy the state in from a device buffer of 36*nx*ny*nz doubles
         (e.g. a torch tensor's data_ptr()) on the context's device."""

@@ -6,7 +6,12 @@ files (``ewpt.io``).  Plots are not produced.
 Command line, the reference's 12 positional ARGS (parameters.jl:14-55)::
 
     python -m ewpt.run gamma no_bubbles nte dx dt_div N gp2 nsnaps T beta_Y beta_W seed_mult \\
-        [--bubbles] [--profile ref|config] [--precision fp64|mixed] [--out DIR] [--fft]
+        [--bubbles] [--profile ref|config] [--precision fp64|mixed] [--out DIR] [--fft] \\
+        [--checkpoint-every K] [--resume]
+
+``--checkpoint-every K`` writes ``DIR/checkpoint.ewpt`` (``ewpt.checkpoint``)
+every K steps; ``--resume`` continues from it, giving bitwise the outputs of
+the uninterrupted run.  The reference has no restart files.
 
 The reference's entry point runs the thermal IC (run(thermal_init=true),
 main_rk4.jl:7082); ``--bubbles`` selects the bubble branch.
@@ -20,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import io
+from . import checkpoint, io
 from ._lib import EWPT_FP64, EWPT_MIXED
 from .ic import bubble_table, profile_id
 from .lattice import Lattice
@@ -36,11 +41,19 @@ def _fft_record(lat: Lattice, p, theta_w: float, path: str):
             w.add(name, np.fft.fftn(em[c].T))
 
 
+# what must agree between a checkpoint and the run that resumes from it
+_RUN_KEYS = ("N", "dx", "dt", "nte", "nsnaps", "gamma_inp", "T", "gp2", "gy", "beta_W", "beta_Y", "no_bubbles",
+             "seed_mult")
+
+
 def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int = EWPT_FP64,
-        out_dir: str | None = ".", write_fft: bool = False, nsnaps_fft: int = 50, verbose: bool = True) -> dict:
+        out_dir: str | None = ".", write_fft: bool = False, nsnaps_fft: int = 50, verbose: bool = True,
+        checkpoint_path: str | None = None, checkpoint_every: int = 0, resume: bool = False) -> dict:
     """Evolve one lattice as run(thermal_init) does; returns the outputs and,
     when out_dir is given, writes data.ewpt (and 3d-fft-data-<it>.ewpt files
-    every dsnaps_fft steps with write_fft)."""
+    every dsnaps_fft steps with write_fft).  With checkpoint_every > 0 the
+    run writes checkpoint_path every that many steps; resume=True continues
+    from checkpoint_path instead of drawing the initial condition."""
     N, dx, dt, mH = P.N, P.dx, P.dt, P.mH
     prof = profile_id(profile)
     theta_w = P.theta_w
@@ -48,38 +61,65 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
     dsnaps_fft = max(1, P.nte // nsnaps_fft)
     if dsnaps < 1:
         raise ValueError("nte must be >= nsnaps")
+    if (checkpoint_every or resume) and not checkpoint_path:
+        raise ValueError("checkpointing needs checkpoint_path")
+    ident = np.array([float(getattr(P, k)) for k in _RUN_KEYS] + [float(thermal_init), float(prof)])
     lat = Lattice(N, dx, precision=precision)
     res: dict = {}
     try:
-        seed = P.base_seed
-        if thermal_init:
-            lat.init_thermal(P.T, P.meff_sq, seed, prof)
-        else:
-            table, hoft = bubble_table(N, P.no_bubbles, seed, prof, bub_diam=P.bub_diam, with_angles=True)
-            lat.init_bubbles(table, prof, mH)
-            res["bubble_table"] = table
-            res["hoft_angles"] = hoft
-
         p0 = P.step_params()
-        energies = np.zeros((P.nsnaps + 1, 13))
-        slices = np.zeros((P.nsnaps + 1, 7, N, N))     # Julia (nsnaps+1, 7, Nx, Nz)
-        energies[0] = lat.energy_row(p0, theta_w)
-        slices[0] = np.transpose(lat.y_slice(p0, theta_w), (0, 2, 1))
-        k0, s0 = lat.spectrum(p0, theta_w)
-        if out_dir and write_fft:
-            _fft_record(lat, p0, theta_w, os.path.join(out_dir, "3d-fft-data-0.ewpt"))
-
-        snp = 0
-        pending = None   # snapshot whose energy columns the next step's stage 1 reduces
         cols = [0, 1, 2, 3, 4, 5, 6, 9, 10, 11]
+        if resume:
+            ck = checkpoint.load(checkpoint_path)
+            if not np.array_equal(np.asarray(ck["run_ident"]), ident):
+                raise ValueError(f"{checkpoint_path} was written by a run with other parameters")
+            it0 = checkpoint.restore(lat, ck)
+            energies, slices = np.array(ck["energies"]), np.array(ck["slices"])
+            k0, s0 = np.array(ck["k0"]), np.array(ck["s0"])
+            snp = int(ck["snp"])
+            pend = np.asarray(ck["pending"])
+            pending = None if pend[0] < 0 else (int(pend[0]), int(pend[1]))
+            if not thermal_init:
+                res["bubble_table"] = np.array(ck["bubble_table"])
+                res["hoft_angles"] = np.array(ck["hoft_angles"]) if "hoft_angles" in ck else None
+            del ck
+        else:
+            it0 = 0
+            seed = P.base_seed
+            if thermal_init:
+                lat.init_thermal(P.T, P.meff_sq, seed, prof)
+            else:
+                table, hoft = bubble_table(N, P.no_bubbles, seed, prof, bub_diam=P.bub_diam, with_angles=True)
+                lat.init_bubbles(table, prof, mH)
+                res["bubble_table"] = table
+                res["hoft_angles"] = hoft
+
+            energies = np.zeros((P.nsnaps + 1, 13))
+            slices = np.zeros((P.nsnaps + 1, 7, N, N))     # Julia (nsnaps+1, 7, Nx, Nz)
+            energies[0] = lat.energy_row(p0, theta_w)
+            slices[0] = np.transpose(lat.y_slice(p0, theta_w), (0, 2, 1))
+            k0, s0 = lat.spectrum(p0, theta_w)
+            if out_dir and write_fft:
+                _fft_record(lat, p0, theta_w, os.path.join(out_dir, "3d-fft-data-0.ewpt"))
+            snp = 0
+            pending = None   # snapshot whose energy columns the next step's stage 1 reduces
         t0 = time.perf_counter()
+
+        def save_checkpoint(it_k):
+            rec = {"run_ident": ident, "energies": energies, "slices": slices, "k0": k0, "s0": s0, "snp": snp,
+                   "pending": np.array(pending if pending is not None else (-1, -1))}
+            if not thermal_init:
+                rec["bubble_table"] = res["bubble_table"]
+                if res["hoft_angles"] is not None:
+                    rec["hoft_angles"] = res["hoft_angles"]
+            checkpoint.save(checkpoint_path, lat, it_k, rec)
 
         def report(k, it_k):
             if verbose:
                 print(f"it={it_k} t*mH={it_k * dt * mH:.3f} E_tot={energies[k, :7].sum():.9e} "
                       f"B={energies[k, 7]:.6e} ({time.perf_counter() - t0:.1f} s)", flush=True)
 
-        for it in range(1, P.nte + 1):
+        for it in range(it0 + 1, P.nte + 1):
             jex = J_ext(it, dt, mH, P.t_sat, thermal_init=thermal_init)
             gam = damp(it, dt, mH, P.t_sat, P.gamma_inp, thermal_init=thermal_init)
             p = P.step_params(jex=jex, gamma=gam)
@@ -99,6 +139,8 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
                 pending = (snp, it)
             if out_dir and write_fft and it % dsnaps_fft == 0:
                 _fft_record(lat, p, theta_w, os.path.join(out_dir, f"3d-fft-data-{it}.ewpt"))
+            if checkpoint_every and it % checkpoint_every == 0 and it < P.nte:
+                save_checkpoint(it)
         if pending is not None:
             energies[pending[0], cols] = lat.energies(P.step_params())[cols]
             report(*pending)
@@ -138,12 +180,16 @@ def main(argv=None) -> int:
     ap.add_argument("--gy", type=float, default=None, help="override the derived g_y")
     ap.add_argument("--out", default=".")
     ap.add_argument("--fft", action="store_true", help="write 3d-fft-data files")
+    ap.add_argument("--checkpoint-every", type=int, default=0, help="write DIR/checkpoint.ewpt every K steps")
+    ap.add_argument("--resume", action="store_true", help="continue from DIR/checkpoint.ewpt")
     a = ap.parse_args(argv)
     over = {} if a.gy is None else {"gy": a.gy}
     P = Parameters.from_args(a.args, **over)
     os.makedirs(a.out, exist_ok=True)
+    ck = os.path.join(a.out, "checkpoint.ewpt") if (a.checkpoint_every or a.resume) else None
     run(P, thermal_init=not a.bubbles, profile=a.profile,
-        precision=EWPT_FP64 if a.precision == "fp64" else EWPT_MIXED, out_dir=a.out, write_fft=a.fft)
+        precision=EWPT_FP64 if a.precision == "fp64" else EWPT_MIXED, out_dir=a.out, write_fft=a.fft,
+        checkpoint_path=ck, checkpoint_every=a.checkpoint_every, resume=a.resume)
     return 0
 
 

@@ -75,6 +75,15 @@ int ewpt_get_state(ewpt_ctx *ctx, double *host_state);
 int ewpt_set_state_device(ewpt_ctx *ctx, const double *dev_state);
 int ewpt_get_state_device(ewpt_ctx *ctx, double *dev_state);
 
+/* The mixed-precision state as the FP32 pair it is evolved in (host buffers
+ * of 36*nx*ny*nz floats each; the state is hi + lo), so that a checkpoint
+ * resumes bit for bit; the FP64 view of ewpt_get_state rounds the pair when
+ * lo is below FP64 resolution of hi.  No counterpart in the reference, which
+ * keeps FP64 state and writes no checkpoint; the state get/set it would bind
+ * is that above.  EWPT_ERR_INVAL on an FP64 context. */
+int ewpt_get_state_pair(ewpt_ctx *ctx, float *host_hi, float *host_lo);
+int ewpt_set_state_pair(ewpt_ctx *ctx, const float *host_hi, const float *host_lo);
+
 /* Measurement aid, no counterpart in the reference: a plain FP64 device copy
  * of n doubles with one 8-byte access per lane (the stage kernels' access
  * width), launched reps + 1 times on stream; *ms = mean time of the last

@@ -107,3 +107,39 @@ def test_cli(tmp_path):
     d = io.read(str(tmp_path / "data.ewpt"))
     assert d["energies"].shape == (3, 13) and int(d["no_bubbles"]) == 2
     assert d["Hoft_angles"].shape == (2, 3)
+
+
+@pytest.mark.parametrize("thermal,precision", [(True, ewpt.EWPT_FP64), (False, ewpt.EWPT_FP64),
+                                               (True, ewpt.EWPT_MIXED)])
+def test_resume_from_checkpoint_is_bitwise(thermal, precision, tmp_path):
+    """A run that writes checkpoints every 6 steps, and a second run resumed
+    from its last checkpoint (step 18, taken right after a snapshot, so the
+    resumed run starts with the step that reduces that snapshot's energies),
+    give bitwise the state, energies and slices of the uninterrupted run."""
+    N, nte, nsnaps = 32, 20, 3           # snapshots every 6 steps
+    P = ewpt.Parameters(N=N, dx=DX, nte=nte, nsnaps=nsnaps, no_bubbles=3, gamma_inp=0.2, T=0.5, gy=0.35)
+    print(f"resume thermal={thermal} precision={precision}", flush=True)
+    ck = str(tmp_path / "checkpoint.ewpt")
+    os.makedirs(tmp_path / "a")
+    full = runmod.run(P, thermal_init=thermal, precision=precision, out_dir=str(tmp_path / "a"),
+                      verbose=False, checkpoint_path=ck, checkpoint_every=6)
+    c = ewpt.checkpoint.load(ck)
+    assert int(c["it"]) == 18 and int(c["snp"]) == 3 and int(np.asarray(c["pending"])[0]) == 3
+    os.makedirs(tmp_path / "b")
+    resumed = runmod.run(P, thermal_init=thermal, precision=precision, out_dir=str(tmp_path / "b"),
+                         verbose=False, checkpoint_path=ck, resume=True)
+    for k in ("energies", "slices", "gauss", "final_state"):
+        assert np.array_equal(full[k], resumed[k]), k
+    # the spectrum's shell sums are FP64 atomics (order-dependent rounding):
+    # the same state gives the same spectrum to rounding, not bit for bit
+    np.testing.assert_allclose(resumed["spectrum"], full["spectrum"], rtol=1e-12, atol=0)
+    a, b = io.read(str(tmp_path / "a" / "data.ewpt")), io.read(str(tmp_path / "b" / "data.ewpt"))
+    assert a.keys() == b.keys()
+    for k in a:
+        if k == "B_linear_fft":
+            np.testing.assert_allclose(b[k], a[k], rtol=1e-12, atol=0)
+        else:
+            assert np.array_equal(a[k], b[k]), k
+    with pytest.raises(ValueError):
+        runmod.run(ewpt.Parameters(N=N, dx=DX, nte=nte + 2, nsnaps=nsnaps, gy=0.35), thermal_init=thermal,
+                   precision=precision, out_dir=None, verbose=False, checkpoint_path=ck, resume=True)
