@@ -539,6 +539,11 @@ def main(argv=None) -> int:
         achieved = canon[dom] * sites / (stage_ms[dom - 1] * 1e-3)
         traffic, source = measured_traffic(precision, active, dom, n)
         if traffic is None and n == 512:
+            if a.variant == 0:
+                # the automatic variant must carry its counter figures: a kernel
+                # that changed under it needs a new counter run first
+                raise SystemExit(f"bench: no measured traffic for the automatic {precision} variant {active} "
+                                 f"(stage {dom}) at 512^3; add its rocprofv3 counters to MEASURED_TRAFFIC")
             print(f"bench: WARNING no measured traffic for {precision} variant {active} stage {dom}: "
                   "roofline.traffic is null", file=sys.stderr, flush=True)
         return {

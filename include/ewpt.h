@@ -182,8 +182,8 @@ int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
 /* nsteps RK4 steps as ewpt_step, and the energy table (ewpt_energies
  * layout: columns 0-6, 9, 11; the EM columns 7, 8, 12 are 0) and the Gauss
  * norms (ewpt_gauss layout) of the state before the first step.  On the
- * FP64 path with stage variant 32 (the automatic one where it tiles) the
- * first step's stage 1 reduces them from the values it stages anyway (the
+ * FP64 path with stage variant 32 or 34 (34 is the automatic one where it
+ * tiles) the first step's stage 1 reduces them from the values it stages anyway (the
  * reductions of main_rk4.jl:6449-6560 and the Gauss residuals of the
  * k_Gamma / k_Sigma charges, wave-reduced in the stage kernel): no extra
  * pass over the state.  Otherwise (mixed path, other variants) the
@@ -323,11 +323,11 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
 /* Kernel variant of the stage launches.  All variants compute the same
  * result (to the rounding of the arithmetic); they exist for the autotuner
  * and for measurements.
- *   0 = automatic: FP64 the first of 332, 324, 325, 320, 311, 310, 210, 211,
- *       2 that tiles the lattice, else 9; mixed the first of 333, 331, 315,
- *       324, 311, 211, 310, 210, 325.  The stored-stage-input variants (15,
- *       24, 31, 32, 33) only when their buffer fits in device memory and not
- *       on slab contexts.
+ *   0 = automatic: FP64 the first of 334, 332, 324, 325, 320, 311, 310, 210,
+ *       211, 2 that tiles the lattice, else 9; mixed the first of 333, 331,
+ *       315, 324, 311, 211, 310, 210, 325.  The stored-stage-input variants
+ *       (15, 24, 31, 32, 33, 34) only when their buffer fits in device memory
+ *       and not on slab contexts.
  *   LDS-brick stage kernels (brick shape, threads, form):
  *       2 (FP64) = 8x8x8, 512 threads, prefetch depth 1;
  *      10 = 16x8x4, 512 threads, every stage input formed on load;
@@ -343,9 +343,14 @@ int ewpt_gauss(ewpt_ctx *ctx, const ewpt_params *p, double out[8]);
  *           a lean stage-4 stash (two workgroups per CU), prefetch 4/4/4/3;
  *      32 (FP64) = 24 with the next input's y loaded during the last passes
  *           and stage 4's update loads spread over its passes (bitwise 24);
- *      33 (mixed) = 31 with the same spread stage 4 (bitwise 31).
- *       15, 24, 31, 32, 33 hold one more state-sized buffer (allocated when
- *       first selected).  Prefetch depth 4/4/3/2 by stage unless stated.
+ *      33 (mixed) = 31 with the same spread stage 4, stages 2-3 storing the
+ *           next input's field channels during their last passes and stage 1
+ *           parking its non-field centres in LDS (bitwise 31);
+ *      34 (FP64) = 32 with y rebuilt instead of read: stage 2 forms the next
+ *           input from u2 - k1 dt/2, stage 4 its update from u4 - k3 dt
+ *           (one more rounding per rebuilt value; 416 B/site less per step).
+ *       15, 24, 31, 32, 33, 34 hold one more state-sized buffer (allocated
+ *       when first selected).  Prefetch depth 4/4/3/2 by stage unless stated.
  *   9 = gather kernel (FP64; one site per thread, any lattice size)
  * A brick variant plus 100*o selects brick enumeration order o (0 = x,y,z;
  * 1 = x,z,y; 2 = x then 4x4 tiles of (y,z) brick rows; 3-9 = cubes of
