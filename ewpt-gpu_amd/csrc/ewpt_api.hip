@@ -212,7 +212,7 @@ int push_view(ewpt_ctx *c)
 // 332 140.6, 324 170.3, 320 181.2, 311 184.4, 310 187.0 ms/step; mixed:
 // 333 78.0, 331 82.7, 315 89.4, 311 96.3, 211 103.2 ms/step), else the
 // gather kernel.  The stored-stage-input variants (334, 333, ...) hold one
-// more state-sized buffer; without room for it (or on a slab) AUTO skips them.
+// more state-sized buffer; without room for it AUTO skips them.
 int resolved_variant(const ewpt_ctx *c)
 {
     if (c->variant != VARIANT_AUTO) return c->variant;
@@ -264,12 +264,25 @@ int launch_stage_kernel(ewpt_ctx *c, int stage, const ewpt_params *p)
 }
 
 // ---- halo exchange of z-slab contexts
-// what = 0: the state (mixed: its FP32 high word, all the stage inputs read);
-// what = s in 1..3: the stage derivative k_s.  Stage s reads at halo sites
-// the state and k_{s-1} (ewpt_stage.h), so the state is exchanged before
-// stage 1 and k_s after stage s.
+// what = 0: the state (mixed: its FP32 high word), which stage 1 reads at
+// halo sites.  what = s in 1..3, before stage s+1: what that stage reads at
+// halo sites -- k_s for the variants that form their input on load from the
+// state and the k's (ewpt_stage.h; the state's and k_1..k_{s-1}'s ghost
+// planes are still current), or the stored stage input u_{s+1} (36
+// channels: u2 and u4 in the stage-input buffer, u3 in the next-state
+// buffer) for the variants that store it; their other reads are interior.
+bool halo_is_stage_input(const ewpt_ctx *c, int what)
+{
+    return what > 0 && brick_stored_inputs(resolved_variant(c));
+}
+
 void *halo_array(ewpt_ctx *c, int what, int &nch)
 {
+    if (halo_is_stage_input(c, what)) {
+        nch = NCH;
+        if (mixed(c)) return what == 2 ? static_cast<void *>(c->fhi_n) : static_cast<void *>(c->fu);
+        return what == 2 ? static_cast<void *>(c->ynext) : static_cast<void *>(c->u);
+    }
     nch = what == 0 ? NCH : NK;
     if (mixed(c)) return what == 0 ? static_cast<void *>(c->fhi) : static_cast<void *>(c->fk[what - 1]);
     return what == 0 ? static_cast<void *>(c->y) : static_cast<void *>(c->k[what - 1]);
@@ -277,13 +290,15 @@ void *halo_array(ewpt_ctx *c, int what, int &nch)
 
 size_t halo_bytes(const ewpt_ctx *c, int what)
 {
-    return (size_t)(what == 0 ? NCH : NK) * HALO * c->plane * (mixed(c) ? sizeof(float) : sizeof(double));
+    const int nch = what == 0 || halo_is_stage_input(c, what) ? NCH : NK;
+    return (size_t)nch * HALO * c->plane * (mixed(c) ? sizeof(float) : sizeof(double));
 }
 
 int halo_pack_into(ewpt_ctx *c, int what, void *lo, void *hi)
 {
     int nch;
     void *a = halo_array(c, what, nch);
+    if (!a) return fail(c, EWPT_ERR_STATE, "halo pack of a stage input before stage 1 wrote it");
     hipError_t e = mixed(c) ? launch_halo_pack(static_cast<const float *>(a), c->n3, c->plane, c->lat.nz, nch,
                                                static_cast<float *>(lo), static_cast<float *>(hi), c->stream)
                             : launch_halo_pack(static_cast<const double *>(a), c->n3, c->plane, c->lat.nz, nch,
@@ -296,6 +311,7 @@ int halo_unpack_from(ewpt_ctx *c, int what, const void *lo, const void *hi)
 {
     int nch;
     void *a = halo_array(c, what, nch);
+    if (!a) return fail(c, EWPT_ERR_STATE, "halo unpack of a stage input before stage 1 wrote it");
     hipError_t e = mixed(c) ? launch_halo_unpack(static_cast<float *>(a), c->n3, c->plane, c->lat.nz, nch,
                                                  static_cast<const float *>(lo), static_cast<const float *>(hi),
                                                  c->stream)
@@ -395,7 +411,6 @@ int create_ctx(ewpt_ctx **out, const ewpt_lattice *lat, int precision, int devic
     c->zg = zg;
     c->plane = (long)lat->nx * lat->ny;
     c->n3 = c->plane * (lat->nz + 2L * zg);
-    if (zg) c->stored_ok = false;   // the exchange carries y and the k's, not stored stage inputs
     int rc = set_device(c);
     if (rc == EWPT_OK && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(c, EWPT_ERR_HIP, "hipStreamCreate");
@@ -898,7 +913,7 @@ namespace {
 int slab_ok(ewpt_ctx *c, int what)
 {
     if (!c->zg) return fail(c, EWPT_ERR_INVAL, "not a slab context");
-    if (what < 0 || what > 3) return fail(c, EWPT_ERR_INVAL, "halo array must be 0 (state) or 1..3 (k_s)");
+    if (what < 0 || what > 3) return fail(c, EWPT_ERR_INVAL, "halo array must be 0 (state) or 1..3 (before stage 2..4)");
     if (!c->has_state) return fail(c, EWPT_ERR_STATE, "halo exchange before set_state");
     return EWPT_OK;
 }
@@ -914,6 +929,9 @@ int ring_ok(ewpt_ctx *const *sl, int n)
         if (sl[i]->device != sl[0]->device || sl[i]->precision != sl[0]->precision ||
             sl[i]->lat.nx != sl[0]->lat.nx || sl[i]->lat.ny != sl[0]->lat.ny || sl[i]->lat.dx != sl[0]->lat.dx)
             return fail(sl[0], EWPT_ERR_INVAL, "slab_step: slabs differ in device, precision, nx, ny or dx");
+        // the exchange carries k_s or the stored stage input, by variant
+        if (brick_stored_inputs(resolved_variant(sl[i])) != brick_stored_inputs(resolved_variant(sl[0])))
+            return fail(sl[0], EWPT_ERR_INVAL, "slab_step: slabs mix variants with and without stored stage inputs");
         if (!sl[i]->has_state) return fail(sl[0], EWPT_ERR_STATE, "slab_step before set_state");
     }
     return EWPT_OK;
@@ -1207,8 +1225,8 @@ int ewpt_set_variant(ewpt_ctx *ctx, int variant)
         ctx->variant = variant;
         return EWPT_OK;
     }
-    if (ctx->zg && (variant == VARIANT_GATHER || brick_stored_inputs(variant)))
-        return fail(ctx, EWPT_ERR_UNSUPPORTED, "slab contexts take the brick variants without stored stage inputs");
+    if (ctx->zg && variant == VARIANT_GATHER)
+        return fail(ctx, EWPT_ERR_UNSUPPORTED, "slab contexts take the brick variants only");
     if (mixed(ctx)) {
         if (!brick_exists_f32(variant)) return EWPT_ERR_INVAL;
         if (!brick_supported_f32(variant, ctx->lat))

@@ -208,8 +208,10 @@ int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps,
  * planes below and above in every array; its stage kernels read the z halo
  * from the ghost planes instead of wrapping.  Before stage s the planes that
  * stage reads at halo sites are exchanged: the state before stage 1 (what =
- * 0; mixed: its FP32 high word) and k_{s-1} before stage s > 1 (what =
- * s-1).  A multi-GPU driver runs, per stage and rank:
+ * 0; mixed: its FP32 high word) and, before stage s > 1 (what = s-1), k_{s-1}
+ * (20 channels) for the variants that form their input on load, or the
+ * stored stage input u_s (36 channels) for the variants that store it (the
+ * automatic choice where it fits).  A multi-GPU driver runs, per stage and rank:
  *   ewpt_slab_pack(ctx, what, send_lo, send_hi)   lowest / highest 3 interior planes
  *   send send_lo to the lower and send_hi to the upper z-neighbour; receive
  *   recv_lo from the lower neighbour (its send_hi), recv_hi from the upper
@@ -219,11 +221,12 @@ int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps,
  * (ewpt_get_stream); buffers are device memory of ewpt_slab_halo_bytes each.
  * ewpt_step on a slab context alone is periodic in z (it exchanges with
  * itself).  Slab contexts take set/get/copy_state, steps, the brick stage
- * variants without stored stage inputs and the calls below; the
+ * variants and the calls below; the
  * whole-lattice calls (ICs, RHS, energies, Gauss, EM fields, spectrum,
  * corrector, batches) return EWPT_ERR_UNSUPPORTED on them. */
 int ewpt_create_slab(ewpt_ctx **out, const ewpt_lattice *local, int precision, int device);
-/* Bytes of one side's halo buffer for array `what` (0 state, 1..3 k_s). */
+/* Bytes of one side's halo buffer for array `what` (0 state, 1..3 before
+ * stage 2..4: k_s or the stored stage input, by the active variant). */
 int ewpt_slab_halo_bytes(ewpt_ctx *ctx, int what, unsigned long long *bytes);
 /* The context's own send buffers (device, large enough for what = 0). */
 int ewpt_slab_buffers(ewpt_ctx *ctx, void **send_lo, void **send_hi);

@@ -30,13 +30,14 @@ def _split(s, n):
     return np.split(s, n, axis=1)          # (36, nz, ny, nx): z is axis 1
 
 
-@pytest.mark.parametrize("precision,variant", [(_lib.EWPT_FP64, 320), (_lib.EWPT_FP64, 10), (_lib.EWPT_MIXED, 311)])
+@pytest.mark.parametrize("precision,variant", [(_lib.EWPT_FP64, 320), (_lib.EWPT_FP64, 10), (_lib.EWPT_MIXED, 311),
+                                               (_lib.EWPT_FP64, 334), (_lib.EWPT_FP64, 324), (_lib.EWPT_MIXED, 333)])
 def test_slab_alone_is_periodic(precision, variant):
     shape = (32, 32, 16)
     print(f"slab alone {shape} precision {precision} variant {variant}", flush=True)
     s = _state(shape, 31)
     with ewpt.Lattice(shape, DX, precision) as a, ewpt.Lattice(shape, DX, precision, slab=True) as b:
-        assert b.active_variant % 100 not in (9, 15, 24, 31, 32, 33)   # no stored stage inputs on a slab
+        assert b.active_variant == (334 if precision == _lib.EWPT_FP64 else 333)   # the automatic choice
         a.variant = variant
         b.variant = variant
         assert a.active_variant == b.active_variant
@@ -50,7 +51,9 @@ def test_slab_alone_is_periodic(precision, variant):
 
 @pytest.mark.parametrize("nslab", [2, 4])
 @pytest.mark.parametrize("precision,variant", [(_lib.EWPT_FP64, 311), (_lib.EWPT_FP64, 2), (_lib.EWPT_FP64, 10),
-                                               (_lib.EWPT_MIXED, 311), (_lib.EWPT_MIXED, 210)])
+                                               (_lib.EWPT_MIXED, 311), (_lib.EWPT_MIXED, 210),
+                                               (_lib.EWPT_FP64, 334), (_lib.EWPT_FP64, 332), (_lib.EWPT_MIXED, 333),
+                                               (_lib.EWPT_MIXED, 315)])
 def test_slabs_equal_one_lattice(nslab, precision, variant):
     full = (32, 32, 32)
     if variant % 100 == 2 and 32 // nslab % 8:
@@ -76,9 +79,11 @@ def test_slabs_equal_one_lattice(nslab, precision, variant):
     assert np.array_equal(got, want), np.abs(got - want).max()
 
 
-def test_exchange_primitives_as_a_multi_gpu_driver():
+@pytest.mark.parametrize("variant", [0, 325])
+def test_exchange_primitives_as_a_multi_gpu_driver(variant):
     """pack -> hand over -> unpack -> stage through the public calls, the
-    way a driver with one rank per GPU would, equals ewpt_slab_step."""
+    way a driver with one rank per GPU would, equals ewpt_slab_step (the
+    automatic variant exchanges the stored stage inputs, 325 the k's)."""
     full = (32, 32, 32)
     s = _state(full, 33)
     parts = _split(s, 2)
@@ -86,7 +91,10 @@ def test_exchange_primitives_as_a_multi_gpu_driver():
     ref = [ewpt.Lattice((32, 32, 16), DX, slab=True) for _ in range(2)]
     try:
         for lat, part in zip(drv + ref, parts + parts):
+            lat.variant = variant
             lat.set_state(part)
+        nk = 36 if variant == 0 else 20
+        assert drv[0].halo_bytes(1) == nk * 3 * 32 * 32 * 8
         bufs = [lat.slab_buffers() for lat in drv]      # (send_lo, send_hi) each
         for _ in range(2):
             for stage in (1, 2, 3, 4):
@@ -135,7 +143,10 @@ def test_pack_holds_the_boundary_planes(precision):
         lat.set_state(s)
         nb = lat.halo_bytes(0)
         assert nb == 36 * 3 * 16 * 8 * word
-        assert lat.halo_bytes(1) == 20 * 3 * 16 * 8 * word
+        assert lat.halo_bytes(1) == 36 * 3 * 16 * 8 * word     # the stored stage input u2
+        lat.variant = 325                                       # 16x4x4, inputs formed on load
+        assert lat.halo_bytes(1) == 20 * 3 * 16 * 8 * word     # k_1
+        lat.variant = 0
         lo, hi = lat.slab_buffers()
         lat.slab_pack(0, lo, hi)
         lat.sync()
@@ -159,10 +170,9 @@ def test_slab_rejects_whole_lattice_calls():
             with pytest.raises(ewpt.EwptError) as ei:
                 call()
             assert ei.value.code == _lib.EWPT_ERR_UNSUPPORTED
-        for v in (9, 24):
-            with pytest.raises(ewpt.EwptError) as ei:
-                lat.variant = v
-            assert ei.value.code == _lib.EWPT_ERR_UNSUPPORTED
+        with pytest.raises(ewpt.EwptError) as ei:
+            lat.variant = 9                                  # the gather kernel wraps z
+        assert ei.value.code == _lib.EWPT_ERR_UNSUPPORTED
         with ewpt.Lattice((16, 16, 8), DX) as periodic:
             periodic.set_state(_state((16, 16, 8), 35))
             with pytest.raises(ewpt.EwptError):
