@@ -1540,7 +1540,7 @@ extern "C" {
 struct ewpt_net {
     Net *net = nullptr;
     int device = 0;
-    double *save = nullptr;   // state before a corrected coarse step (fallback)
+    double *save[2] = {nullptr, nullptr};   // states before the last two coarse steps (fallback)
     long save_n = 0;
     std::string err;
 };
@@ -1601,7 +1601,8 @@ int ewpt_net_destroy(ewpt_net *n)
     if (!n) return EWPT_ERR_INVAL;
     (void)hipSetDevice(n->device);
     net_destroy(n->net);
-    if (n->save) (void)hipFree(n->save);
+    for (double *b : n->save)
+        if (b) (void)hipFree(b);
     delete n;
     return EWPT_OK;
 }
@@ -1696,41 +1697,87 @@ int ewpt_step_corrected(ewpt_ctx *ctx, ewpt_net *n, const ewpt_params *p, int fa
     int rc = net_ctx_ok(n, ctx);
     if (rc) return rc;
     if (hipSetDevice(n->device) != hipSuccess) return EWPT_ERR_HIP;
-    const long need = NCH * ctx->n3;
-    if (n->save_n != need) {
-        if (n->save) (void)hipFree(n->save);
-        n->save = nullptr;
-        n->save_n = 0;
-        if (hipMalloc(reinterpret_cast<void **>(&n->save), need * sizeof(double)) != hipSuccess)
-            return nfail(n, EWPT_ERR_OOM, "net: alloc fallback copy");
-        n->save_n = need;
-    }
     ewpt_params pc = *p;
     pc.dt = p->dt * factor;
-    int fb = 0;
     const bool check = !std::isinf(gauss_tol) || gauss_tol < 0;   // +inf: never fall back, no checks
-    double g0 = 0.0, e0 = 0.0;
-    if (check && (rc = gauss_and_enorm(ctx, p, &g0, &e0))) return rc;
+    if (!check) {
+        for (int it = 0; it < ncoarse; it++) {
+            if ((rc = ewpt_step(ctx, &pc, 1))) return rc;
+            if ((rc = ewpt_net_apply(n, ctx))) return rc;
+        }
+        if (nfallback) *nfallback = 0;
+        return EWPT_OK;
+    }
+    const long need = NCH * ctx->n3;
+    if (n->save_n != need) {
+        for (double *&b : n->save) {
+            if (b) (void)hipFree(b);
+            b = nullptr;
+        }
+        n->save_n = 0;
+        for (double *&b : n->save)
+            if (hipMalloc(reinterpret_cast<void **>(&b), need * sizeof(double)) != hipSuccess)
+                return nfail(n, EWPT_ERR_OOM, "net: alloc fallback copies");
+        n->save_n = need;
+    }
+    // The Gauss residual norm G of a corrected state is reduced inside stage 1
+    // of the next coarse step (ewpt_step_diag), so the check costs no extra
+    // pass over the lattice; only the last corrected state takes the
+    // standalone reduction.  A state whose drift (G - G_prev) / |E| exceeds
+    // the tolerance is replaced by `factor` fine steps from the accepted state
+    // before its coarse step (save[base]), and the coarse step that had
+    // started from it is redone.  The decisions and states are those of
+    // checking every corrected state right after its correction.
+    auto copy = [&](double *dst, const double *src) {
+        return hipMemcpyAsync(dst, src, need * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+    };
+    // one coarse step with the Gauss norm G and |E| of its input state
+    auto coarse = [&](double *G, double *En) {
+        double e[13], g[8];
+        const int r = ewpt_step_diag(ctx, &pc, 1, e, g);
+        *G = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3]);
+        *En = std::sqrt(2.0 * (e[3] + e[5]));
+        return r;
+    };
+    // replace the state by `factor` fine steps from save[b]
+    auto fine_from = [&](int b) {
+        HIPCHK(ctx, copy(ctx->y, n->save[b]), "restore state");
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "restore sync");
+        return ewpt_step(ctx, p, factor);
+    };
+    int fb = 0, base = 0;   // save[base]: the accepted state before the pending coarse step
+    double g0 = 0.0;        // G of the last accepted state
     for (int it = 0; it < ncoarse; it++) {
-        double g1 = 0.0, e1 = 0.0;
-        if (check)
-            HIPCHK(ctx, hipMemcpyAsync(n->save, ctx->y, need * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
-                   "save state");
-        if ((rc = ewpt_step(ctx, &pc, 1))) return rc;
+        double G = 0.0, En = 0.0;
+        HIPCHK(ctx, copy(n->save[base ^ 1], ctx->y), "save state");
+        rc = coarse(&G, &En);
+        if (rc && rc != EWPT_ERR_NONFINITE) return rc;
+        if (it == 0) {
+            if (rc) {   // the input state itself is not finite: leave it as it was
+                HIPCHK(ctx, copy(ctx->y, n->save[base ^ 1]), "restore state");
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "restore sync");
+                return rc;
+            }
+        } else if (rc == EWPT_ERR_NONFINITE || !((G - g0) / std::max(En, 1e-12) <= gauss_tol)) {
+            // the corrected state of the previous coarse step failed: redo that
+            // step with fine steps, then this coarse step from the result
+            if ((rc = fine_from(base))) return rc;
+            fb++;
+            HIPCHK(ctx, copy(n->save[base ^ 1], ctx->y), "save state");
+            if ((rc = coarse(&G, &En))) return rc;
+        }
+        g0 = G;
+        base ^= 1;
         if ((rc = ewpt_net_apply(n, ctx))) return rc;
-        if (!check) continue;
-        if ((rc = gauss_and_enorm(ctx, p, &g1, &e1)) && rc != EWPT_ERR_NONFINITE) return rc;
-        const double drift = (g1 - g0) / std::max(e1, 1e-12);
-        if (rc == EWPT_ERR_NONFINITE || !(drift <= gauss_tol)) {
-            // the constraint error grew past the tolerance: redo with fine steps
-            HIPCHK(ctx, hipMemcpyAsync(ctx->y, n->save, need * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream),
-                   "restore state");
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "restore sync");
-            if ((rc = ewpt_step(ctx, p, factor))) return rc;
-            if ((rc = gauss_and_enorm(ctx, p, &g1, &e1))) return rc;
+    }
+    if (ncoarse > 0) {
+        double G = 0.0, En = 0.0;
+        rc = gauss_and_enorm(ctx, p, &G, &En);
+        if (rc && rc != EWPT_ERR_NONFINITE) return rc;
+        if (rc == EWPT_ERR_NONFINITE || !((G - g0) / std::max(En, 1e-12) <= gauss_tol)) {
+            if ((rc = fine_from(base))) return rc;
             fb++;
         }
-        g0 = g1;   // the residual after this step is the next step's reference
     }
     if (nfallback) *nfallback = fb;
     return EWPT_OK;

@@ -28,6 +28,7 @@
 #include <mutex>
 #include <new>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "ewpt_internal.h"
@@ -45,6 +46,7 @@ constexpr int TROW = BRN + 8; // LDS row pitch of a transposed tile
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short f2bf(float f)
 {
@@ -59,7 +61,11 @@ __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z
 
 __device__ __forceinline__ int wrapn(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-enum : int { EPI_HIDDEN = 0, EPI_LAST = 1, EPI_DGRAD = 2 };
+enum : int { EPI_HIDDEN = 0, EPI_LAST = 1, EPI_DGRAD = 2, EPI_APPLY = 3 };
+
+// bf16 of f, round to nearest even (v_cvt_pk_bf16_f32); the values f2bf gives
+// for every finite f and for the infinities
+__device__ __forceinline__ unsigned short bf16_rne(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
 // X: [n3][64] bf16; Wk: [27][64 n][64 k] bf16; out per epilogue:
 //   EPI_HIDDEN: z = acc + bias -> Z (f32, optional), A = bf16(silu(z))
@@ -168,6 +174,221 @@ __global__ void __launch_bounds__(256) conv_kernel(const unsigned short *__restr
                 }
             }
         }
+}
+
+// conv2_kernel: the same convolution with the weights kept out of LDS.  A
+// workgroup of 2 waves (two workgroups per CU) owns an 8x8x4 brick; its
+// input plus a 1-site halo (600 sites x 128 B) is loaded once, straight from
+// global memory into LDS, as 128-byte rows whose 16-byte chunks are permuted
+// by the low 3 bits of the site index (the 8 rows one LDS cycle serves fall
+// on distinct banks).  Wave w computes the 128 sites
+// of z-planes 2w, 2w+1 against NB*16 outputs: 8 x NB accumulator tiles
+// (16x16, FP32) over 27 taps x 2 k-steps.  The B fragments (one tap's
+// weights, 16 bytes per lane per tile, identical for every workgroup and so
+// served by L1/L2) go straight into a ring of three register sets, two taps
+// ahead of their use; with 128 sites per wave each weight byte feeds twice
+// the matrix work of a 64-site tile, which keeps the L1 path at half its
+// rate.  The A fragments are read from LDS, written once, so the tap loop
+// has no barrier.  The outputs leave through LDS as whole 16-byte chunks.
+// NB = 3 computes outputs 0..47 only (the last layer: its outputs 36..63
+// are padding and its Z rows are written up to channel 47).
+constexpr int C2X = 8, C2Y = 8, C2Z = 4, C2N = C2X * C2Y * C2Z;
+constexpr int C2HX = C2X + 2, C2HY = C2Y + 2, C2HZ = C2Z + 2, C2HN = C2HX * C2HY * C2HZ;
+#ifndef EWPT_CONV_WAVES
+#define EWPT_CONV_WAVES 2
+#endif
+constexpr int C2W = EWPT_CONV_WAVES;   // waves per workgroup
+constexpr int C2MB = 16 / C2W;         // 16-site M-blocks per wave (the brick's 256 sites over C2W waves)
+constexpr int C2T = 64 * C2W;          // threads per workgroup
+constexpr size_t CONV2_LDS = (size_t)C2HN * NC * sizeof(unsigned short);
+
+// bf16 offset of 16-byte chunk c (0..7) of halo site h in the conv2 tile
+__device__ __forceinline__ int c2_off(int h, int c) { return h * NC + ((c ^ (h & 7)) << 3); }
+
+template <int EPI, int NB>
+__global__ void __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(C2W == 2 ? 1 : 2, C2W == 2 ? 1 : 2)))
+conv2_kernel(const unsigned short *__restrict__ X, const unsigned short *__restrict__ Wk,
+             const float *__restrict__ bias, const float *__restrict__ zprev, int nx, int ny, int nz,
+             float *__restrict__ Z, unsigned short *__restrict__ A, double *__restrict__ Y,
+             const float *__restrict__ sout)
+{
+    extern __shared__ __align__(16) unsigned short lds16[];
+    unsigned short *tile = lds16;   // [C2HN] rows of 64 bf16, chunks permuted (c2_off)
+    const int nbx = nx / C2X, nby = ny / C2Y, nbr = nbx * nby * (nz / C2Z);
+    int bid = blockIdx.x;
+    // consecutive block ids go to different XCDs: give XCD x a contiguous
+    // range of bricks so that the halo rows neighbours share stay in its L2
+    if ((nbr & 7) == 0) bid = (bid & 7) * (nbr >> 3) + (bid >> 3);
+    const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+    const int x0 = bx * C2X, y0 = by * C2Y, z0 = bz * C2Z;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, r16 = lane & 15, q = lane >> 4;
+
+    auto loadB = [&](int tap, bf16x8 (&b)[NB][2]) {
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+                b[nb][ks] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4 *>(Wk + ((long)(tap * NC + nb * 16 + r16) * NC + ks * 32 + q * 8)));
+    };
+    bf16x8 bA[NB][2], bB[NB][2], bC[NB][2];
+    loadB(0, bA);
+    loadB(1, bB);
+
+    // input brick + halo straight from global memory into LDS
+    // (global_load_lds_dwordx4: lane l of a wave instruction writes the 16
+    // bytes it loads to LDS slot base + l), so every chunk of the tile is in
+    // flight at once without passing through registers.  Slot k = 8 h + p of
+    // halo site h holds its chunk p ^ (h & 7) (the c2_off permutation).
+    {
+        constexpr int NINST = C2HN * 8 / 64;
+        static_assert(C2HN * 8 % 64 == 0, "the tile is a whole number of 64-slot wave loads");
+        for (int i = w; i < NINST; i += C2W) {
+            const int k = i * 64 + lane, h = k >> 3, c = (k & 7) ^ (h & 7);
+            const int hx = h % C2HX, hy = (h / C2HX) % C2HY, hz = h / (C2HX * C2HY);
+            const long g =
+                wrapn(x0 + hx - 1, nx) + (long)nx * (wrapn(y0 + hy - 1, ny) + (long)ny * wrapn(z0 + hz - 1, nz));
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(X + g * NC + c * 8),
+                                             (__attribute__((address_space(3))) void *)(tile + i * 64 * 8), 16, 0, 0);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the tile has landed in LDS
+    }
+    __syncthreads();
+
+    f32x4 acc[C2MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < C2MB; mb++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // A-fragment row r16 of M-block mb is site (r16 & 7, 2 (m & 3) + (r16 >> 3)) of z-plane m >> 2,
+    // m = C2MB w + mb the M-block's index in the brick
+    int hb[C2MB];
+#pragma unroll
+    for (int mb = 0; mb < C2MB; mb++) {
+        const int m = C2MB * w + mb;
+        hb[mb] = ((m >> 2) + 1) * (C2HX * C2HY) + (2 * (m & 3) + (r16 >> 3) + 1) * C2HX + (r16 & 7) + 1;
+    }
+    auto step = [&](int tap, const bf16x8 (&b)[NB][2]) {
+        const int off = (tap / 9 - 1) * (C2HX * C2HY) + ((tap / 3) % 3 - 1) * C2HX + (tap % 3 - 1);
+        bf16x8 a[C2MB][2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int mb = 0; mb < C2MB; mb++)
+                a[mb][ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(tile + c2_off(hb[mb] + off, ks * 4 + q)));
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int mb = 0; mb < C2MB; mb++)
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb][ks], b[nb][ks], acc[mb][nb], 0, 0, 0);
+    };
+    for (int tap = 0; tap < NTAP; tap += 3) {
+        loadB(tap + 2, bC);
+        step(tap, bA);
+        if (tap + 3 < NTAP) loadB(tap + 3, bA);
+        step(tap + 1, bB);
+        if (tap + 4 < NTAP) loadB(tap + 4, bB);
+        step(tap + 2, bC);
+    }
+
+    // epilogue through LDS: lane holds rows q*4 + i of each M-block and column
+    // r16 of each N-block; the brick's rows are staged in the tile region
+    // (row = site s = x + 8 y + 64 z of the brick) and written as 16-byte chunks
+    __syncthreads();   // every wave is done with the input tile
+    auto site_of = [&](int mb, int i) {
+        const int r = q * 4 + i, m = C2MB * w + mb;
+        return (r & 7) + C2X * (2 * (m & 3) + (r >> 3)) + C2X * C2Y * (m >> 2);
+    };
+    auto gsite = [&](int s) {
+        return (long)(x0 + (s & 7)) + (long)nx * ((y0 + ((s >> 3) & 7)) + (long)ny * (z0 + (s >> 6)));
+    };
+    constexpr int NO = NB * 16;   // output channels written per site
+    if constexpr (EPI == EPI_APPLY) {
+        // the correction y[c] += sout[c] * (acc + bias)[c], c < 36, into the
+        // FP64 state (channel-major), in the order apply_corr adds it
+        float *zs = reinterpret_cast<float *>(lds16);
+        constexpr int ZP = NO + 4;
+#pragma unroll
+        for (int mb = 0; mb < C2MB; mb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    zs[site_of(mb, i) * ZP + nb * 16 + r16] = acc[mb][nb][i] + bias[nb * 16 + r16];
+        __syncthreads();
+        // 72 elements per thread in batches of 24 loads in flight (loads and
+        // stores of y share a pointer, so the compiler keeps their order)
+        const long n3 = (long)nx * ny * nz;
+        constexpr int NE = NCH * C2N / C2T, YB = 24;
+        static_assert(NCH * C2N % C2T == 0 && NE % YB == 0, "whole batches");
+        for (int e0 = 0; e0 < NE; e0 += YB) {
+            double v[YB];
+#pragma unroll
+            for (int j = 0; j < YB; j++) {
+                const int e = t + (e0 + j) * C2T, c = e / C2N, s = e % C2N;
+                v[j] = Y[c * n3 + gsite(s)] + (double)sout[c] * (double)zs[s * ZP + c];
+            }
+#pragma unroll
+            for (int j = 0; j < YB; j++) {
+                const int e = t + (e0 + j) * C2T;
+                Y[(e / C2N) * n3 + gsite(e % C2N)] = v[j];
+            }
+        }
+        return;
+    }
+    if constexpr (EPI != EPI_DGRAD) {
+        if (Z) {
+            // FP32 pre-activations, row pitch NO + 4 floats
+            float *zs = reinterpret_cast<float *>(lds16);
+            constexpr int ZP = NO + 4;
+#pragma unroll
+            for (int mb = 0; mb < C2MB; mb++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int nb = 0; nb < NB; nb++)
+                        zs[site_of(mb, i) * ZP + nb * 16 + r16] = acc[mb][nb][i] + bias[nb * 16 + r16];
+            __syncthreads();
+            constexpr int ZCH = NO / 4;   // 16-byte chunks per site
+            for (int c = t; c < C2N * ZCH; c += C2T) {
+                const int s = c / ZCH, part = c % ZCH;
+                *reinterpret_cast<float4 *>(Z + gsite(s) * NC + part * 4) =
+                    *reinterpret_cast<const float4 *>(zs + s * ZP + part * 4);
+            }
+            __syncthreads();
+        }
+    }
+    if constexpr (EPI != EPI_LAST) {
+        // bf16 activations (or input gradients), row pitch 72
+        constexpr int AP = NC + 8;
+#pragma unroll
+        for (int mb = 0; mb < C2MB; mb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int s = site_of(mb, i);
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++) {
+                    const int co = nb * 16 + r16;
+                    float v;
+                    if constexpr (EPI == EPI_HIDDEN) {
+                        const float zz = acc[mb][nb][i] + bias[co];
+                        v = zz * sigm(zz);
+                    } else {
+                        const float zp = zprev[gsite(s) * NC + co];
+                        const float sg = sigm(zp);
+                        v = acc[mb][nb][i] * (sg * (1.0f + zp * (1.0f - sg)));
+                    }
+                    lds16[s * AP + co] = bf16_rne(v);
+                }
+            }
+        __syncthreads();
+        for (int c = t; c < C2N * 8; c += C2T) {
+            const int s = c >> 3, part = c & 7;
+            *reinterpret_cast<uint4 *>(A + gsite(s) * NC + part * 8) = *reinterpret_cast<const uint4 *>(lds16 + s * AP + part * 8);
+        }
+    }
 }
 
 // per chunk of bricks and per tap: partial[chunk][tap][out][in]; the tap-0
@@ -406,23 +627,39 @@ unsigned grid_of(long n, long per = 256)
 
 constexpr size_t CONV_LDS = (size_t)(HN * ROW + 2 * NC * ROW) * sizeof(unsigned short);
 
-template <int EPI>
+// NB: output blocks of 16 the caller needs (4: all 64 channels; 3: the last
+// layer's 36 state channels, in 48); the 8x4x4-brick kernel computes all 64
+bool conv2_fits(int nx, int ny, int nz) { return nx % C2X == 0 && ny % C2Y == 0 && nz % C2Z == 0; }
+
+// EPI_APPLY (conv2 lattices only): y += sout * (last layer's output)
+template <int EPI, int NB = 4>
 hipError_t conv_launch(const unsigned short *X, const unsigned short *Wk, const float *bias, const float *zprev,
-                       int nx, int ny, int nz, float *Z, unsigned short *A, hipStream_t st)
+                       int nx, int ny, int nz, float *Z, unsigned short *A, hipStream_t st, double *Y = nullptr,
+                       const float *sout = nullptr)
 {
     static std::mutex attr_mu;   // several threads may launch the first convolution at once
     static bool attr = false;
     {
         std::lock_guard<std::mutex> g(attr_mu);
         if (!attr) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<EPI>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS);
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_kernel<EPI == EPI_APPLY ? EPI_LAST : EPI>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(reinterpret_cast<const void *>(&conv2_kernel<EPI, NB>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV2_LDS);
             if (e != hipSuccess) return e;
             attr = true;
         }
     }
-    const int nb = (nx / BRX) * (ny / BRY) * (nz / BRZ);
-    conv_kernel<EPI><<<nb, 256, CONV_LDS, st>>>(X, Wk, bias, zprev, nx, ny, nz, Z, A);
+    if (conv2_fits(nx, ny, nz)) {
+        const int nb = (nx / C2X) * (ny / C2Y) * (nz / C2Z);
+        conv2_kernel<EPI, NB><<<nb, C2T, CONV2_LDS, st>>>(X, Wk, bias, zprev, nx, ny, nz, Z, A, Y, sout);
+    } else if constexpr (EPI == EPI_APPLY) {
+        return hipErrorInvalidValue;
+    } else {
+        const int nb = (nx / BRX) * (ny / BRY) * (nz / BRZ);
+        conv_kernel<EPI><<<nb, 256, CONV_LDS, st>>>(X, Wk, bias, zprev, nx, ny, nz, Z, A);
+    }
     return hipGetLastError();
 }
 
@@ -504,8 +741,10 @@ hipError_t refresh_weights(Net *n)
     return hipGetLastError();
 }
 
-// forward of the three layers from X0; training keeps Z1, Z2
-hipError_t forward(Net *n, const ewpt_lattice &L, bool keep)
+// forward of the three layers from X0; training keeps Z1, Z2.  With y set
+// (and a lattice the brick kernel tiles) the last layer adds the correction
+// to y instead of writing Z3.
+hipError_t forward(Net *n, const ewpt_lattice &L, bool keep, double *y = nullptr)
 {
     NetBufs &b = n->buf;
     hipError_t e = conv_launch<EPI_HIDDEN>(b.X0, n->Wb, Bl(n, 0), nullptr, L.nx, L.ny, L.nz, keep ? b.Z1 : nullptr,
@@ -513,8 +752,11 @@ hipError_t forward(Net *n, const ewpt_lattice &L, bool keep)
     if (e == hipSuccess)
         e = conv_launch<EPI_HIDDEN>(b.A1, n->Wb + NW, Bl(n, 1), nullptr, L.nx, L.ny, L.nz, keep ? b.Z2 : nullptr, b.A2,
                                     n->st);
+    if (e == hipSuccess && y)
+        return conv_launch<EPI_APPLY, 3>(b.A2, n->Wb + 2 * NW, Bl(n, 2), nullptr, L.nx, L.ny, L.nz, nullptr, nullptr,
+                                         n->st, y, n->sout);
     if (e == hipSuccess)
-        e = conv_launch<EPI_LAST>(b.A2, n->Wb + 2 * NW, Bl(n, 2), nullptr, L.nx, L.ny, L.nz, b.Z3, nullptr, n->st);
+        e = conv_launch<EPI_LAST, 3>(b.A2, n->Wb + 2 * NW, Bl(n, 2), nullptr, L.nx, L.ny, L.nz, b.Z3, nullptr, n->st);
     return e;
 }
 
@@ -611,10 +853,14 @@ hipError_t net_apply(Net *n, double *y, const ewpt_lattice &L)
     const long n3 = n->buf.n3;
     if (e != hipSuccess) return e;
     pack_state<<<grid_of(n3), 256, 0, n->st>>>(y, n3, L.nx, L.ny, L.nz, n->input_mode, n->sin_inv, n->buf.X0);
-    e = forward(n, L, false);
-    if (e == hipSuccess) {
-        apply_corr<<<grid_of(n3), 256, 0, n->st>>>(y, n3, n->buf.Z3, n->sout);
-        e = hipGetLastError();
+    if (conv2_fits(L.nx, L.ny, L.nz)) {
+        e = forward(n, L, false, y);   // the correction is added by the last layer
+    } else {
+        e = forward(n, L, false);
+        if (e == hipSuccess) {
+            apply_corr<<<grid_of(n3), 256, 0, n->st>>>(y, n3, n->buf.Z3, n->sout);
+            e = hipGetLastError();
+        }
     }
     return e == hipSuccess ? hipStreamSynchronize(n->st) : e;
 }

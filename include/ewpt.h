@@ -444,7 +444,11 @@ int ewpt_net_train_step(ewpt_net *n, ewpt_ctx *coarse, ewpt_ctx *fine, double lr
  * correction.  When the Gauss-law residual norm sqrt(sum G^2) grows over a
  * coarse step by more than gauss_tol * ||E|| (||E||^2 = 2 (EEW + EEY)), the
  * step is redone as factor fine steps; *nfallback counts those
- * (gauss_tol = +inf never falls back and skips the residual evaluations). */
+ * (gauss_tol = +inf never falls back and skips the residual evaluations).
+ * The residual of each corrected state is reduced inside stage 1 of the next
+ * coarse step where the stage variant has the diagnostic stage 1 (FP64
+ * contexts); the context then keeps two saved states (2 x 36 N^3 doubles on
+ * the corrector) to redo a failed step. */
 int ewpt_step_corrected(ewpt_ctx *ctx, ewpt_net *n, const ewpt_params *p, int factor, int ncoarse,
                         double gauss_tol, int *nfallback);
 /* Test hooks on host buffers: one convolution layer (mode 0: hidden layer,

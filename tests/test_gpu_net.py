@@ -131,6 +131,36 @@ def test_training_gradients_match_oracle():
         assert np.linalg.norm(g[sl] - gref[sl]) < 3e-2 * np.linalg.norm(gref[sl]), l
 
 
+@pytest.mark.parametrize("shape", [SHAPE, (16, 12, 8)])   # 8x8x4 brick kernel; 8x4x4 kernel + apply_corr
+def test_apply_matches_oracle(shape):
+    """y + sout * net(y / in_scale) on the device (the last layer adds the
+    correction to the FP64 state in its epilogue where the lattice tiles
+    8x8x4) against the oracle's forward pass with the same bf16 roundings."""
+    c, _ = _pair(shape, 9)
+    P = net_np.init_params(3).astype(np.float32)
+    rng = np.random.default_rng(4)
+    o = 2 * (net_np.NW + net_np.NC)
+    P[o:] = (0.05 * rng.standard_normal(net_np.NW + net_np.NC)).astype(np.float32)   # a nonzero last layer
+    in_scale = (0.5 + rng.random(36)).astype(np.float32)
+    out_scale = (1e-3 * (0.5 + rng.random(36))).astype(np.float32)
+    with ewpt.Lattice(shape, 0.25) as lat, ewpt.Corrector(seed=1) as net:
+        net.params = P
+        net.set_norm(in_scale, out_scale)
+        net.input_mode = 0
+        lat.set_state(c)
+        net.apply(lat)
+        got = lat.get_state() - c
+    inv = (np.float32(1.0) / in_scale).astype(np.float64)
+    X0 = np.zeros(c.shape[1:] + (64,))
+    X0[..., :36] = np.moveaxis(c * inv[:, None, None, None], 0, -1)
+    Z3 = net_np.forward(P.astype(np.float64), net_np.bf16(X0), emulate=True)["Z3"]
+    want = np.moveaxis(Z3[..., :36], -1, 0) * out_scale.astype(np.float64)[:, None, None, None]
+    assert np.abs(want).max() > 0
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert rel < 1e-2, rel
+    assert np.abs(got - want).max() <= 2.0 ** -6 * np.abs(want).max(), np.abs(got - want).max()
+
+
 def test_step_corrected_fallback_paths():
     shape = (32, 16, 16)
     p = fields.params(dt=0.025)
@@ -160,6 +190,68 @@ def test_step_corrected_fallback_paths():
             b.set_state(s)
             b.step(p4, 1)
             assert not np.array_equal(got, b.get_state())   # the correction did something
+
+
+def _checked_reference(lat, net, p, p4, factor, ncoarse, tol):
+    """The corrected stepping with the Gauss check evaluated right after each
+    correction by the standalone reductions: the decisions ewpt_step_corrected
+    must reproduce.  Returns (fallbacks, drifts)."""
+    def norms():
+        g, e = lat.gauss(p), lat.energies(p)
+        return math.sqrt(float(np.sum(g[:4] ** 2))), math.sqrt(2.0 * (e[3] + e[5]))
+    g0, _ = norms()
+    fb, drifts = 0, []
+    for _ in range(ncoarse):
+        save = lat.get_state()
+        lat.step(p4, 1)
+        net.apply(lat)
+        g1, e1 = norms()
+        drifts.append((g1 - g0) / max(e1, 1e-12))
+        if not drifts[-1] <= tol:
+            lat.set_state(save)
+            lat.step(p, factor)
+            g1, _ = norms()
+            fb += 1
+        g0 = g1
+    return fb, drifts
+
+
+def test_step_corrected_checks_match_reference():
+    """Checks taken from the fused stage-1 reductions of the next coarse step
+    give the fallbacks and states of checking each corrected state at once:
+    all accepted, and a tolerance between the drifts that sends some steps
+    back to fine steps."""
+    shape = (32, 16, 16)
+    p = fields.params(dt=0.025)
+    p4 = dict(p, dt=0.1)
+    s = fields.smooth_state(shape, seed=5, amp=0.3)
+    ncoarse = 6
+    with ewpt.Corrector(seed=2) as net:
+        rng = np.random.default_rng(1)
+        P = net.params
+        P[2 * (net_np.NW + net_np.NC):] = 3e-3 * rng.standard_normal(net_np.NW + net_np.NC)
+        net.params = P
+        net.set_norm(np.ones(36), 1e-3 * np.ones(36))
+        with ewpt.Lattice(shape, 0.25) as a, ewpt.Lattice(shape, 0.25) as b:
+            b.set_state(s)
+            fb, drifts = _checked_reference(b, net, p, p4, 4, ncoarse, 1e30)
+            assert fb == 0
+            a.set_state(s)
+            assert net.step(a, p, factor=4, ncoarse=ncoarse, gauss_tol=1e30) == 0
+            assert np.array_equal(a.get_state(), b.get_state())
+            # a tolerance well inside the spread of the drifts
+            d = np.sort(np.asarray(drifts))
+            gaps = np.diff(d)
+            k = int(np.argmax(gaps))
+            assert gaps[k] > 1e-6 * np.abs(d).max()
+            tol = 0.5 * (d[k] + d[k + 1])
+            b.set_state(s)
+            fb, drifts2 = _checked_reference(b, net, p, p4, 4, ncoarse, tol)
+            assert fb >= 1
+            assert min(abs(x - tol) for x in drifts2) > 1e-9 * np.abs(d).max()
+            a.set_state(s)
+            assert net.step(a, p, factor=4, ncoarse=ncoarse, gauss_tol=tol) == fb
+            assert np.array_equal(a.get_state(), b.get_state())
 
 
 def test_trained_corrector_reduces_coarse_error():
