@@ -510,7 +510,8 @@ def main(argv=None) -> int:
                     "step_diag_ms": wall(lambda: lat.step_diag(P, 1)),
                     "standalone_energies_gauss_ms": wall(lambda: (lat.energies(P), lat.gauss(P)))}
             snap["fused_overhead_frac"] = snap["step_diag_ms"] / snap["step_ms"] - 1.0
-            snap["fused"] = precision == "fp64" and active % 100 in (32, 34)   # variants with the diagnostic stage 1
+            # variants with the diagnostic stage 1 (FP64 32/34, mixed 31/33)
+            snap["fused"] = active % 100 in ((32, 34) if precision == "fp64" else (31, 33))
             done_snap = 8             # wall() runs each stepping call 1 + 3 times
             done = a.warmup + a.steps + done_snap
             if a.precision_steps:

@@ -226,6 +226,23 @@ int resolved_variant(const ewpt_ctx *c)
     return VARIANT_GATHER;
 }
 
+// the mixed path's stage buffers (FP32 pair, k's, stored stage input)
+StageBufs<float> fbufs_for(ewpt_ctx *c, int stage, bool us)
+{
+    StageBufs<float> b{};
+    b.y = c->fhi;
+    b.ylo = c->flo;
+    b.k1 = c->fk[0];
+    b.k2 = c->fk[1];
+    b.k3 = c->fk[2];
+    b.n3 = c->n3;
+    if (stage < 4) b.kout = c->fk[stage - 1];
+    else { b.ynew = c->fhi_n; b.ylo_new = c->flo_n; }
+    if (us) set_stage_inputs(b, stage, c->fu, c->fhi_n);
+    b.zghost = c->zg;
+    return b;
+}
+
 int launch_stage_kernel(ewpt_ctx *c, int stage, const ewpt_params *p)
 {
     const Phys P = phys_of(*p);
@@ -236,18 +253,8 @@ int launch_stage_kernel(ewpt_ctx *c, int stage, const ewpt_params *p)
         if (rc) return rc;
     }
     if (mixed(c)) {
-        StageBufs<float> b{};
-        b.y = c->fhi;
-        b.ylo = c->flo;
-        b.k1 = c->fk[0];
-        b.k2 = c->fk[1];
-        b.k3 = c->fk[2];
-        b.n3 = c->n3;
-        if (stage < 4) b.kout = c->fk[stage - 1];
-        else { b.ynew = c->fhi_n; b.ylo_new = c->flo_n; }
-        if (us) set_stage_inputs(b, stage, c->fu, c->fhi_n);
-        b.zghost = c->zg;
-        HIPCHK(c, launch_stage_brick_f32(v, stage, b, c->lat, P, p->dt, c->stream), "stage launch");
+        HIPCHK(c, launch_stage_brick_f32(v, stage, fbufs_for(c, stage, us), c->lat, P, p->dt, c->stream),
+               "stage launch");
         if (stage == 4) {
             std::swap(c->fhi, c->fhi_n);
             std::swap(c->flo, c->flo_n);
@@ -834,7 +841,8 @@ int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energ
     int rc = set_device(ctx);
     if (rc) return rc;
     const int v = resolved_variant(ctx);
-    if (mixed(ctx) || !brick_has_diag(v)) {
+    const bool fdiag = mixed(ctx) && brick_has_diag_f32(v) && brick_supported_f32(v, ctx->lat);
+    if (!fdiag && (mixed(ctx) || !brick_has_diag(v))) {
         // no diagnostic stage 1 for this precision / variant: the standalone
         // reductions of the state, then the steps
         const int re = ewpt_energies(ctx, p, energies);
@@ -844,14 +852,23 @@ int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energ
         if ((rc = ewpt_step(ctx, p, nsteps))) return rc;
         return re ? re : rg;
     }
-    const long rows = diag_rows(ctx->lat);
-    if (!ctx->diag && (rc = dev_alloc(ctx, &ctx->diag, (size_t)(rows + 1) * DIAG_SLOTS, "alloc diagnostics")))
+    // rows: one per brick of the diagnostic stage 1 (the FP64 kernels' 16x4x4
+    // bricks outnumber the mixed kernels' 32x4x4 ones, so one allocation
+    // serves both precisions)
+    const long rows = fdiag ? diag_rows_f32(ctx->lat) : diag_rows(ctx->lat);
+    if (!ctx->diag &&
+        (rc = dev_alloc(ctx, &ctx->diag, (size_t)(diag_rows(ctx->lat) + 1) * DIAG_SLOTS, "alloc diagnostics")))
         return rc;
     if ((rc = ensure_stage_input_buffer(ctx, v))) return rc;
     double *res = ctx->diag + rows * DIAG_SLOTS;
-    HIPCHK(ctx, launch_stage1_diag(v, bufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
-                                   ctx->diag, ctx->stream),
-           "diagnostic stage launch");
+    if (fdiag)
+        HIPCHK(ctx, launch_stage1_diag_f32(v, fbufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
+                                           ctx->diag, ctx->stream),
+               "diagnostic stage launch");
+    else
+        HIPCHK(ctx, launch_stage1_diag(v, bufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
+                                       ctx->diag, ctx->stream),
+               "diagnostic stage launch");
     HIPCHK(ctx, launch_finish_rows(ctx->diag, rows, DIAG_SLOTS, 1u << 8, res, ctx->stream), "diagnostic reduction");
     double r[DIAG_SLOTS];
     HIPCHK(ctx, hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, ctx->stream), "diagnostic copy");

@@ -525,7 +525,11 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     //   5 div Y, then (C^Y)^2   6-8 div W^a, then (C^a)^2
     //   9 PE  10 KE  11 |phi|  12 EEW  13 EEY  14 MEY  15 MEW
     constexpr bool DG = DIAG && S == 1;
-    static_assert(!DIAG || (std::is_same_v<T, double> && S == 1 && !PARK), "diagnostics: FP64 stage 1");
+    static_assert(!DIAG || (S == 1 && !PARK), "diagnostics: stage 1 without parked centres");
+    // the FP32 (mixed) form computes each site's densities in FP64 from the
+    // FP32 centre values and pass derivatives (|phi|^2 - v^2 without FP32
+    // cancellation); the FP64 form keeps its own arithmetic
+    auto dd = [](R v) { return (double)v; };
     constexpr int NKD = 16;
     double *dgw = reinterpret_cast<double *>(lds_raw + 2 * B::BUF * sizeof(T));
     double *dgl = dgw + (B::NT / 64) * NDIAG;
@@ -535,11 +539,20 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     if constexpr (DG) {
 #pragma unroll
         for (int k = 0; k < 9; k++) dput(k, 0.0);
-        const R X = r2 - R(P.vev) * R(P.vev);
-        dput(9, R(P.lam) * (X * X));
-        dput(10, c[CH_PI] * c[CH_PI] + c[CH_PI + 1] * c[CH_PI + 1] + c[CH_PI + 2] * c[CH_PI + 2] +
-                     c[CH_PI + 3] * c[CH_PI + 3]);
-        dput(11, sqrt(r2));
+        if constexpr (MIXED) {
+            const double r2d = dd(p[0]) * dd(p[0]) + dd(p[1]) * dd(p[1]) + dd(p[2]) * dd(p[2]) + dd(p[3]) * dd(p[3]);
+            const double X = r2d - P.vev * P.vev;
+            dput(9, P.lam * (X * X));
+            dput(10, dd(c[CH_PI]) * dd(c[CH_PI]) + dd(c[CH_PI + 1]) * dd(c[CH_PI + 1]) +
+                         dd(c[CH_PI + 2]) * dd(c[CH_PI + 2]) + dd(c[CH_PI + 3]) * dd(c[CH_PI + 3]));
+            dput(11, sqrt(r2d));
+        } else {
+            const R X = r2 - R(P.vev) * R(P.vev);
+            dput(9, R(P.lam) * (X * X));
+            dput(10, c[CH_PI] * c[CH_PI] + c[CH_PI + 1] * c[CH_PI + 1] + c[CH_PI + 2] * c[CH_PI + 2] +
+                         c[CH_PI + 3] * c[CH_PI + 3]);
+            dput(11, sqrt(r2));
+        }
     }
 
     // ---- staging of one channel into LDS: centre from registers, faces from
@@ -670,7 +683,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                     kG[aa] += gp2 * gw * na;
                 }
             }
-            if constexpr (DG && pp == 11) {
+            if constexpr (DG && pp == 11 && !MIXED) {
                 // every E centre and div E is known: EEW, EEY and the Gauss residuals
                 R ee = R(0);
 #pragma unroll
@@ -688,6 +701,27 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                     dput(aa, r * r);
                 }
                 const R rY = dget(3) - gy * phiTY(p, piv);
+                dput(3, rY * rY);
+            } else if constexpr (DG && pp == 11) {
+                // the same in FP64 from the FP32 values
+                double ee = 0.0;
+#pragma unroll
+                for (int qq = 0; qq < 9; qq++) ee += dd(c[CH_E + qq]) * dd(c[CH_E + qq]);
+                dput(12, 0.5 * ee);
+                dput(13, 0.5 * (dd(c[CH_EY]) * dd(c[CH_EY]) + dd(c[CH_EY + 1]) * dd(c[CH_EY + 1]) +
+                                dd(c[CH_EY + 2]) * dd(c[CH_EY + 2])));
+                const double pd[4] = {dd(p[0]), dd(p[1]), dd(p[2]), dd(p[3])};
+                const double piv[4] = {dd(c[CH_PI]), dd(c[CH_PI + 1]), dd(c[CH_PI + 2]), dd(c[CH_PI + 3])};
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) {
+                    double na = 0.0;
+#pragma unroll
+                    for (int ii = 0; ii < 3; ii++)
+                        na -= xprod(aa, dd(W(0, ii)), dd(W(1, ii)), dd(W(2, ii)), dd(E(0, ii)), dd(E(1, ii)), dd(E(2, ii)));
+                    const double r = dget(aa) - P.gw * (na + phiT(aa, pd, piv));
+                    dput(aa, r * r);
+                }
+                const double rY = dget(3) - P.gy * phiTY(pd, piv);
                 dput(3, rY * rY);
             }
         } else if constexpr (kind == 1) {
@@ -724,7 +758,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 R Ap[4];
                 apply_gen(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], p, Ap);
                 const R D = d1 + Ap[bb];
-                if constexpr (DG) dacc(4, D * D);
+                if constexpr (DG) dacc(4, dd(D) * dd(D));
                 add_gen_col<bb>(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], d1 + D, ktphi);
                 ktW[0][i] += gw * phiT_coef<0, bb>(p) * D;
                 ktW[1][i] += gw * phiT_coef<1, bb>(p) * D;
@@ -753,8 +787,8 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             }
             ktY[i] += lap;
             if constexpr (DG && pp == 22) {
-                dput(14, R(0.5) * (BYv[0] * BYv[0] + BYv[1] * BYv[1] + BYv[2] * BYv[2]));
-                const R cY = c[CH_SIG] - dget(5);
+                dput(14, 0.5 * (dd(BYv[0]) * dd(BYv[0]) + dd(BYv[1]) * dd(BYv[1]) + dd(BYv[2]) * dd(BYv[2])));
+                const double cY = dd(c[CH_SIG]) - dget(5);
                 dput(5, cY * cY);
             }
             if constexpr (BETA && pp == 22) {
@@ -799,13 +833,14 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             }
             ktW[a][i] += lap;
             if constexpr (DG && pp == 31) {
-                R me = R(0);
+                double me = 0.0;
 #pragma unroll
-                for (int aa = 0; aa < 3; aa++) me += Bw[aa][0] * Bw[aa][0] + Bw[aa][1] * Bw[aa][1] + Bw[aa][2] * Bw[aa][2];
-                dput(15, R(0.5) * me);
+                for (int aa = 0; aa < 3; aa++)
+                    me += dd(Bw[aa][0]) * dd(Bw[aa][0]) + dd(Bw[aa][1]) * dd(Bw[aa][1]) + dd(Bw[aa][2]) * dd(Bw[aa][2]);
+                dput(15, 0.5 * me);
 #pragma unroll
                 for (int aa = 0; aa < 3; aa++) {
-                    const R cc = c[CH_GAM + aa] - dget(6 + aa);
+                    const double cc = dd(c[CH_GAM + aa]) - dget(6 + aa);
                     dput(6 + aa, cc * cc);
                 }
             }
@@ -1128,10 +1163,12 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 }
 
 // Stage 1 with the diagnostics of its input (the state) reduced on the way:
-// per brick NDIAG partial sums to diag[brick * NDIAG + slot].
-template <int TX, int TY, int TZ, int PF, bool BETA>
+// per brick NDIAG partial sums to diag[brick * NDIAG + slot].  Two waves per
+// SIMD (the per-site slots in LDS leave room for two 256-thread or one
+// 512-thread workgroup per CU).
+template <int TX, int TY, int TZ, int PF, bool BETA, typename T = double>
 __global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(waves_per_simd(false, PF, 1))))
-stage1_diag(StageBufs<double> b, BrickGeo g, Phys P, double dt, double *diag)
+stage1_diag(StageBufs<T> b, BrickGeo g, Phys P, T dt, double *diag)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     int bid = blockIdx.x;
@@ -1139,7 +1176,7 @@ stage1_diag(StageBufs<double> b, BrickGeo g, Phys P, double dt, double *diag)
         const int per = (g.nbx * g.nby * g.nbz) >> 3;
         bid = (bid & 7) * per + (bid >> 3);
     }
-    brick_body<TX, TY, TZ, PF, BETA, 1, double, true>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag);
+    brick_body<TX, TY, TZ, PF, BETA, 1, T, true>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag);
 }
 
 template <int TX, int TY, int TZ>
@@ -1187,24 +1224,24 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
     return hipGetLastError();
 }
 
-template <int TX, int TY, int TZ, int PF, bool BETA>
-hipError_t launch_diag_beta(const StageBufs<double> &b, const BrickGeo &g, const Phys &P, double dt, double *diag,
+template <int TX, int TY, int TZ, int PF, bool BETA, typename T>
+hipError_t launch_diag_beta(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, double *diag,
                             hipStream_t st)
 {
     using B = BrickShape<TX, TY, TZ>;
-    const size_t lds = 2 * B::BUF * sizeof(double) + ((B::NT / 64) * NDIAG + 16 * B::NT) * sizeof(double);
+    const size_t lds = 2 * B::BUF * sizeof(T) + ((B::NT / 64) * NDIAG + 16 * B::NT) * sizeof(double);
     static std::mutex attr_mu;
     static bool attr_set = false;
     {
         std::lock_guard<std::mutex> gd(attr_mu);
         if (!attr_set) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage1_diag<TX, TY, TZ, PF, BETA>),
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage1_diag<TX, TY, TZ, PF, BETA, T>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             attr_set = true;
         }
     }
-    stage1_diag<TX, TY, TZ, PF, BETA><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag);
+    stage1_diag<TX, TY, TZ, PF, BETA, T><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag);
     return hipGetLastError();
 }
 
@@ -1223,13 +1260,13 @@ BrickGeo make_brick_geo(int TX, int TY, int TZ, const ewpt_lattice &L, int xcd_m
 }
 
 // stage 1 with the diagnostics (diag: one NDIAG row per brick)
-template <int TX, int TY, int TZ, int PF>
-hipError_t launch_tile_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt, int xcd_map,
+template <int TX, int TY, int TZ, int PF, typename T = double>
+hipError_t launch_tile_diag(const StageBufs<T> &b, const ewpt_lattice &L, const Phys &P, double dt, int xcd_map,
                             int order, double *diag, hipStream_t st)
 {
     const BrickGeo g = make_brick_geo(TX, TY, TZ, L, xcd_map, order, b.zghost);
-    if (P.bw != 0.0 || P.by != 0.0) return launch_diag_beta<TX, TY, TZ, PF, true>(b, g, P, dt, diag, st);
-    return launch_diag_beta<TX, TY, TZ, PF, false>(b, g, P, dt, diag, st);
+    if (P.bw != 0.0 || P.by != 0.0) return launch_diag_beta<TX, TY, TZ, PF, true, T>(b, g, P, T(dt), diag, st);
+    return launch_diag_beta<TX, TY, TZ, PF, false, T>(b, g, P, T(dt), diag, st);
 }
 
 template <int TX, int TY, int TZ, int PF, typename T = double>

@@ -56,4 +56,22 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
     }
 }
 
+// Stage 1 of variants 31 and 33 (the same kernel up to where the non-field
+// centres wait: in registers or parked in LDS; bitwise equal outputs) with
+// the energy table and Gauss norms of its input reduced on the way, one row
+// per 32x4x4 brick.
+bool brick_has_diag_f32(int variant)
+{
+    return brick_exists_f32(variant) && (variant % 100 == 31 || variant % 100 == 33);
+}
+
+long diag_rows_f32(const ewpt_lattice &L) { return (long)(L.nx / 32) * (L.ny / 4) * (L.nz / 4); }
+
+hipError_t launch_stage1_diag_f32(int variant, const StageBufs<float> &b, const ewpt_lattice &L, const Phys &P,
+                                  double dt, double *diag, hipStream_t st)
+{
+    if (!brick_has_diag_f32(variant) || !brick_supported_f32(variant, L)) return hipErrorInvalidValue;
+    return launch_tile_diag<32, 4, 4, 1010704443, float>(b, L, P, dt, 1, variant / 100, diag, st);
+}
+
 }  // namespace ewpt

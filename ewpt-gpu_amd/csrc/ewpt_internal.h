@@ -60,7 +60,7 @@ hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b
 // stage 1 that also reduces the energy densities and Gauss residuals of its
 // input (the state): one row of DIAG_SLOTS partial sums per brick in diag
 // (slots: PE KE GE EEW MEW EEY MEY sum|phi| min|phi| non-finite, then
-// (G^a)^2 (G^Y)^2 (C^a)^2 (C^Y)^2).  FP64 variant 32 (+100*order) only.
+// (G^a)^2 (G^Y)^2 (C^a)^2 (C^Y)^2).  FP64 variants 32, 34 (+100*order).
 constexpr int DIAG_SLOTS = 18;
 bool brick_has_diag(int variant);
 long diag_rows(const ewpt_lattice &L);   // rows of diag (bricks of the diagnostic stage 1)
@@ -74,6 +74,12 @@ bool brick_exists_f32(int variant);
 bool brick_supported_f32(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float> &b, const ewpt_lattice &L,
                                   const Phys &P, double dt, hipStream_t st);
+// mixed variants 31, 33 (+100*order): stage 1 with the diagnostics as above,
+// densities in FP64 from the FP32 high words and pass derivatives
+bool brick_has_diag_f32(int variant);
+long diag_rows_f32(const ewpt_lattice &L);
+hipError_t launch_stage1_diag_f32(int variant, const StageBufs<float> &b, const ewpt_lattice &L, const Phys &P,
+                                  double dt, double *diag, hipStream_t st);
 
 // ewpt_mixed.hip
 hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream_t st);

@@ -186,9 +186,13 @@ int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
  * tiles) the first step's stage 1 reduces them from the values it stages anyway (the
  * reductions of main_rk4.jl:6449-6560 and the Gauss residuals of the
  * k_Gamma / k_Sigma charges, wave-reduced in the stage kernel): no extra
- * pass over the state.  Otherwise (mixed path, other variants) the
- * standalone reductions run first.  Equal to ewpt_energies / ewpt_gauss to
- * rounding (the sums are grouped differently).  A snapshot loop calls it
+ * pass over the state; equal to ewpt_energies / ewpt_gauss to rounding (the
+ * sums are grouped differently).  On the mixed path with variant 31 or 33
+ * (33 is the automatic one) the FP32 stage 1 does the same, the densities
+ * computed in FP64 from the FP32 high words and the FP32 pass derivatives:
+ * equal to the FP64 reductions of the state (high + compensation word) to
+ * the FP32 rounding of the fields (~1e-7 relative).  Otherwise (other
+ * variants) the standalone reductions run first.  A snapshot loop calls it
  * for the step after the snapshot.  Returns EWPT_ERR_NONFINITE (after
  * stepping and filling the outputs) if a density is NaN/Inf. */
 int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energies[13], double gauss[8]);
