@@ -194,10 +194,10 @@ __global__ void __launch_bounds__(256) conv_kernel(const unsigned short *__restr
 // are padding and its Z rows are written up to channel 47).
 constexpr int C2X = 8, C2Y = 8, C2Z = 4, C2N = C2X * C2Y * C2Z;
 constexpr int C2HX = C2X + 2, C2HY = C2Y + 2, C2HZ = C2Z + 2, C2HN = C2HX * C2HY * C2HZ;
-#ifndef EWPT_CONV_WAVES
-#define EWPT_CONV_WAVES 2
-#endif
-constexpr int C2W = EWPT_CONV_WAVES;   // waves per workgroup
+// waves per workgroup: 2 (128 sites per wave, one wave per SIMD); 4 waves of
+// 64 sites (two per SIMD) move twice the weight bytes per MFMA and measured
+// slower at 256^3 (6.7 against 5.3 ms per hidden layer)
+constexpr int C2W = 2;
 constexpr int C2MB = 16 / C2W;         // 16-site M-blocks per wave (the brick's 256 sites over C2W waves)
 constexpr int C2T = 64 * C2W;          // threads per workgroup
 constexpr size_t CONV2_LDS = (size_t)C2HN * NC * sizeof(unsigned short);
@@ -206,7 +206,7 @@ constexpr size_t CONV2_LDS = (size_t)C2HN * NC * sizeof(unsigned short);
 __device__ __forceinline__ int c2_off(int h, int c) { return h * NC + ((c ^ (h & 7)) << 3); }
 
 template <int EPI, int NB>
-__global__ void __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(C2W == 2 ? 1 : 2, C2W == 2 ? 1 : 2)))
+__global__ void __launch_bounds__(C2T) __attribute__((amdgpu_waves_per_eu(1, 1)))
 conv2_kernel(const unsigned short *__restrict__ X, const unsigned short *__restrict__ Wk,
              const float *__restrict__ bias, const float *__restrict__ zprev, int nx, int ny, int nz,
              float *__restrict__ Z, unsigned short *__restrict__ A, double *__restrict__ Y,
