@@ -496,19 +496,22 @@ def main(argv=None) -> int:
             stage_ms, total_ms = lat.step_timed(P, a.steps)   # synchronises before returning
             active = lat.active_variant
             e_timed = lat.energies(P)
-            # snapshot cost: a step that also reduces the energy table and Gauss
-            # norms of its input (fused into stage 1 where the variant has it)
-            # against a plain step and the standalone reductions (wall clock,
-            # each call synchronous; mean of 3 after one warm call)
+            # snapshot cost: a step that also reduces the whole energy row (EM
+            # columns included) and the Gauss norms of its input (fused into
+            # stage 1 where the variant has it) against a plain step and the
+            # standalone reductions (wall clock, each call synchronous; mean of
+            # 3 after one warm call)
             def wall(f, n=3):
                 f()
                 t0 = time.perf_counter()
                 for _ in range(n):
                     f()
                 return (time.perf_counter() - t0) / n * 1e3
+            th = params.theta_w
             snap = {"step_ms": wall(lambda: lat.step(P, 1)),
-                    "step_diag_ms": wall(lambda: lat.step_diag(P, 1)),
-                    "standalone_energies_gauss_ms": wall(lambda: (lat.energies(P), lat.gauss(P)))}
+                    "step_diag_ms": wall(lambda: lat.step_diag_em(P, th, 1)),
+                    "standalone_energies_gauss_em_ms": wall(lambda: (lat.energies(P), lat.gauss(P),
+                                                                     lat.em_energies(P, th)))}
             snap["fused_overhead_frac"] = snap["step_diag_ms"] / snap["step_ms"] - 1.0
             # variants with the diagnostic stage 1 (FP64 32/34, mixed 31/33)
             snap["fused"] = active % 100 in ((32, 34) if precision == "fp64" else (31, 33))

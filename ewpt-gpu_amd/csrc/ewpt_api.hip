@@ -819,23 +819,31 @@ int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps)
 
 namespace {
 
-// the energy table (ewpt_energies layout) and Gauss norms (ewpt_gauss layout)
-// from the DIAG_SLOTS reduced sums of the diagnostic stage 1
-void diag_outputs(const ewpt_ctx *c, const double *r, double e[13], double gs[8])
+// the energy table (ewpt_energies layout; with em the EM columns 7, 8, 12
+// of ewpt_em_energies) and Gauss norms (ewpt_gauss layout) from the
+// DIAG_SLOTS reduced sums of the diagnostic stage 1
+void diag_outputs(const ewpt_ctx *c, const double *r, bool em, double e[13], double gs[8])
 {
     const double vol = c->lat.dx * c->lat.dx * c->lat.dx;
     for (int q = 0; q < 13; q++) e[q] = 0.0;
     for (int q = 0; q < 7; q++) e[q] = r[q] * vol;
     e[9] = r[8];
     e[11] = r[7] / (double)sites(c);
+    if (em) {
+        e[7] = 0.5 * r[18] * vol;
+        e[8] = 0.5 * r[19] * vol;
+        e[12] = r[20] * vol;
+    }
     for (int q = 0; q < 8; q++) gs[q] = std::sqrt(r[10 + q] * vol);
 }
 
-}  // namespace
-
-int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energies[13], double gauss[8])
+// nsteps steps with the diagnostics of the input state reduced by the first
+// stage 1 (theta_w: the Weinberg angle of the EM columns when em)
+int step_diag_impl(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, bool em, double theta_w, double energies[13],
+                   double gauss[8])
 {
-    if (!ctx || !params_ok(p) || nsteps < 1 || !energies || !gauss) return EWPT_ERR_INVAL;
+    if (!ctx || !params_ok(p) || nsteps < 1 || !energies || !gauss || (em && !std::isfinite(theta_w)))
+        return EWPT_ERR_INVAL;
     if (ctx->zg) return no_slab(ctx, "step_diag");
     if (!ctx->has_state) return fail(ctx, EWPT_ERR_STATE, "step before set_state");
     int rc = set_device(ctx);
@@ -849,8 +857,17 @@ int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energ
         if (re && re != EWPT_ERR_NONFINITE) return re;
         const int rg = ewpt_gauss(ctx, p, gauss);
         if (rg && rg != EWPT_ERR_NONFINITE) return rg;
+        int rm = EWPT_OK;
+        if (em) {
+            double m[3];
+            rm = ewpt_em_energies(ctx, p, theta_w, m);
+            if (rm && rm != EWPT_ERR_NONFINITE) return rm;
+            energies[7] = m[0];
+            energies[8] = m[1];
+            energies[12] = m[2];
+        }
         if ((rc = ewpt_step(ctx, p, nsteps))) return rc;
-        return re ? re : rg;
+        return re ? re : rg ? rg : rm;
     }
     // rows: one per brick of the diagnostic stage 1 (the FP64 kernels' 16x4x4
     // bricks outnumber the mixed kernels' 32x4x4 ones, so one allocation
@@ -861,13 +878,14 @@ int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energ
         return rc;
     if ((rc = ensure_stage_input_buffer(ctx, v))) return rc;
     double *res = ctx->diag + rows * DIAG_SLOTS;
+    const double sw = em ? std::sin(theta_w) : 0.0, cw = em ? std::cos(theta_w) : 0.0;
     if (fdiag)
         HIPCHK(ctx, launch_stage1_diag_f32(v, fbufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
-                                           ctx->diag, ctx->stream),
+                                           ctx->diag, sw, cw, ctx->stream),
                "diagnostic stage launch");
     else
         HIPCHK(ctx, launch_stage1_diag(v, bufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
-                                       ctx->diag, ctx->stream),
+                                       ctx->diag, sw, cw, ctx->stream),
                "diagnostic stage launch");
     HIPCHK(ctx, launch_finish_rows(ctx->diag, rows, DIAG_SLOTS, 1u << 8, res, ctx->stream), "diagnostic reduction");
     double r[DIAG_SLOTS];
@@ -878,11 +896,24 @@ int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energ
         for (int s = 1; s <= 4; s++)
             if ((rc = launch_stage_kernel(ctx, s, p))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "step sync");
-    diag_outputs(ctx, r, energies, gauss);
+    diag_outputs(ctx, r, em, energies, gauss);
     bool bad = r[9] != 0.0;
     for (int q = 0; q < 8; q++) bad |= !std::isfinite(gauss[q]);
-    if (bad) return fail(ctx, EWPT_ERR_NONFINITE, "energy density or Gauss residual is NaN/Inf");
+    if (bad) return fail(ctx, EWPT_ERR_NONFINITE, "energy density, EM density or Gauss residual is NaN/Inf");
     return EWPT_OK;
+}
+
+}  // namespace
+
+int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energies[13], double gauss[8])
+{
+    return step_diag_impl(ctx, p, nsteps, false, 0.0, energies, gauss);
+}
+
+int ewpt_step_diag_em(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int nsteps, double energies[13],
+                      double gauss[8])
+{
+    return step_diag_impl(ctx, p, nsteps, true, theta_w, energies, gauss);
 }
 
 int ewpt_step_timed(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double stage_ms[4], double *total_ms)

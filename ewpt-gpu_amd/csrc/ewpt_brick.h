@@ -347,12 +347,15 @@ __host__ __device__ constexpr int waves_per_simd(bool f32, int PF, int S) { retu
 // kernel stage_brick and by the batched kernels stage_brick_batch.
 // Slots of the diagnostics a stage-1 brick can reduce on the way (DIAG):
 // the energy densities of the reference's compute_* kernels
-// (main_rk4.jl:1334-2301) and the Gauss-law residuals of the k_Gamma /
-// k_Sigma charges (main_rk4.jl:603-657), in the layout of the standalone
-// reductions: 0 PE, 1 KE, 2 GE, 3 EEW, 4 MEW, 5 EEY, 6 MEY, 7 sum|phi|,
-// 8 min|phi|, 9 non-finite count, 10-12 (G^a)^2, 13 (G^Y)^2, 14-16 (C^a)^2,
-// 17 (C^Y)^2.
-constexpr int NDIAG = 18;
+// (main_rk4.jl:1334-2301), the Gauss-law residuals of the k_Gamma /
+// k_Sigma charges (main_rk4.jl:603-657) and the electromagnetic densities
+// of the magnetic field B, its B_2 form and B_2.A (the EM columns of the
+// energy table, main_rk4.jl:2303-2822, 3718-4250; em_point in ewpt_point.h),
+// in the layout of the standalone reductions: 0 PE, 1 KE, 2 GE, 3 EEW,
+// 4 MEW, 5 EEY, 6 MEY, 7 sum|phi|, 8 min|phi|, 9 non-finite count,
+// 10-12 (G^a)^2, 13 (G^Y)^2, 14-16 (C^a)^2, 17 (C^Y)^2, 18 |B|^2,
+// 19 |B_2|^2, 20 B_2.A.
+constexpr int NDIAG = 21;
 
 __device__ __forceinline__ double wave_reduce(double v, bool mn)
 {
@@ -366,7 +369,8 @@ __device__ __forceinline__ double wave_reduce(double v, bool mn)
 
 template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T, bool DIAG = false>
 __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, int bid,
-                                           unsigned char *lds_raw, int t, double *diag = nullptr)
+                                           unsigned char *lds_raw, int t, double *diag = nullptr, double sw = 0.0,
+                                           double cw = 0.0)
 {
     // T = float is the mixed-precision path: y is stored as an FP32 pair
     // (b.y = high word, b.ylo = compensation word), stages read the high
@@ -515,22 +519,27 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         for (int ch = CH_PI; ch < CH_PI + 4; ch++) park[(ch - 16) * B::NT + t] = c[ch];   // never staged
     }
 
-    // ---- diagnostics (DIAG, stage 1 of the FP64 path).  Every per-site
-    // value is parked in the thread's own LDS slots dgl[k][t] the moment it is
-    // known (the derivative sums div E^a, div E^Y, |D phi|^2, div Y, div W^a
-    // accumulate there beside the right-hand side's); after the passes, when
-    // the registers are free, each slot is wave-reduced (dgw[wave][slot]) and
-    // the waves combined in a fixed order.  Per-thread slots k:
+    // ---- diagnostics (DIAG, stage 1).  Every per-site value that needs the
+    // derivatives is parked in the thread's own LDS slots dgl[k][t] the moment
+    // it is known (the derivative sums div E^a, div E^Y, |D phi|^2, div Y,
+    // div W^a accumulate there beside the right-hand side's); the densities of
+    // centre values alone (PE, KE, |phi|, EEW, EEY) are formed after the
+    // passes from the centres the next stage input needs anyway.  After the
+    // passes each slot is wave-reduced (dgw[wave][slot]) and the waves
+    // combined in a fixed order.  Per-thread slots k:
     //   0-3 div E^a, div E^Y, then (G^a)^2, (G^Y)^2   4 |D phi|^2
-    //   5 div Y, then (C^Y)^2   6-8 div W^a, then (C^a)^2
-    //   9 PE  10 KE  11 |phi|  12 EEW  13 EEY  14 MEY  15 MEW
+    //   5 div Y, then (C^Y)^2   6-8 div W^a, then (C^a)^2   9 MEY  10 MEW
+    //   11-13 D_i phi_1 (pass 16), then the cross products X_i of the B field
+    //         (passes 17, 19), then |B|^2, |B_2|^2, B_2.A (pass 31)
+    //   14-16 D_i phi_3 (pass 18), then B^Y_i (pass 22)
+    // sw, cw: sin and cos of the Weinberg angle of the EM densities.
     constexpr bool DG = DIAG && S == 1;
     static_assert(!DIAG || (S == 1 && !PARK), "diagnostics: stage 1 without parked centres");
     // the FP32 (mixed) form computes each site's densities in FP64 from the
     // FP32 centre values and pass derivatives (|phi|^2 - v^2 without FP32
     // cancellation); the FP64 form keeps its own arithmetic
     auto dd = [](R v) { return (double)v; };
-    constexpr int NKD = 16;
+    constexpr int NKD = 17;
     double *dgw = reinterpret_cast<double *>(lds_raw + 2 * B::BUF * sizeof(T));
     double *dgl = dgw + (B::NT / 64) * NDIAG;
     auto dput = [&](int k, double v) { dgl[k * B::NT + t] = v; };
@@ -539,20 +548,6 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     if constexpr (DG) {
 #pragma unroll
         for (int k = 0; k < 9; k++) dput(k, 0.0);
-        if constexpr (MIXED) {
-            const double r2d = dd(p[0]) * dd(p[0]) + dd(p[1]) * dd(p[1]) + dd(p[2]) * dd(p[2]) + dd(p[3]) * dd(p[3]);
-            const double X = r2d - P.vev * P.vev;
-            dput(9, P.lam * (X * X));
-            dput(10, dd(c[CH_PI]) * dd(c[CH_PI]) + dd(c[CH_PI + 1]) * dd(c[CH_PI + 1]) +
-                         dd(c[CH_PI + 2]) * dd(c[CH_PI + 2]) + dd(c[CH_PI + 3]) * dd(c[CH_PI + 3]));
-            dput(11, sqrt(r2d));
-        } else {
-            const R X = r2 - R(P.vev) * R(P.vev);
-            dput(9, R(P.lam) * (X * X));
-            dput(10, c[CH_PI] * c[CH_PI] + c[CH_PI + 1] * c[CH_PI + 1] + c[CH_PI + 2] * c[CH_PI + 2] +
-                         c[CH_PI + 3] * c[CH_PI + 3]);
-            dput(11, sqrt(r2));
-        }
     }
 
     // ---- staging of one channel into LDS: centre from registers, faces from
@@ -684,12 +679,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 }
             }
             if constexpr (DG && pp == 11 && !MIXED) {
-                // every E centre and div E is known: EEW, EEY and the Gauss residuals
-                R ee = R(0);
-#pragma unroll
-                for (int qq = 0; qq < 9; qq++) ee += c[CH_E + qq] * c[CH_E + qq];
-                dput(12, R(0.5) * ee);
-                dput(13, R(0.5) * (c[CH_EY] * c[CH_EY] + c[CH_EY + 1] * c[CH_EY + 1] + c[CH_EY + 2] * c[CH_EY + 2]));
+                // every E centre and div E is known: the Gauss residuals
                 const R piv[4] = {c[CH_PI], c[CH_PI + 1], c[CH_PI + 2], c[CH_PI + 3]};
 #pragma unroll
                 for (int aa = 0; aa < 3; aa++) {
@@ -704,12 +694,6 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 dput(3, rY * rY);
             } else if constexpr (DG && pp == 11) {
                 // the same in FP64 from the FP32 values
-                double ee = 0.0;
-#pragma unroll
-                for (int qq = 0; qq < 9; qq++) ee += dd(c[CH_E + qq]) * dd(c[CH_E + qq]);
-                dput(12, 0.5 * ee);
-                dput(13, 0.5 * (dd(c[CH_EY]) * dd(c[CH_EY]) + dd(c[CH_EY + 1]) * dd(c[CH_EY + 1]) +
-                                dd(c[CH_EY + 2]) * dd(c[CH_EY + 2])));
                 const double pd[4] = {dd(p[0]), dd(p[1]), dd(p[2]), dd(p[3])};
                 const double piv[4] = {dd(c[CH_PI]), dd(c[CH_PI + 1]), dd(c[CH_PI + 2]), dd(c[CH_PI + 3])};
 #pragma unroll
@@ -750,6 +734,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             // phi_b: kt_phi (Laplacian, A_i(d_i phi + D_i phi)), phi.T D into kt_W, kt_Y, u_j
             constexpr int bb = pp - 16;
             R lap = R(0);
+            [[maybe_unused]] double Dk[3];   // D_i phi_bb, for the B field's cross products
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 R d1, d2;
@@ -758,7 +743,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 R Ap[4];
                 apply_gen(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], p, Ap);
                 const R D = d1 + Ap[bb];
-                if constexpr (DG) dacc(4, dd(D) * dd(D));
+                if constexpr (DG) {
+                    dacc(4, dd(D) * dd(D));
+                    Dk[i] = dd(D);
+                }
                 add_gen_col<bb>(h, W(0, i), W(1, i), W(2, i), hy, c[CH_Y + i], d1 + D, ktphi);
                 ktW[0][i] += gw * phiT_coef<0, bb>(p) * D;
                 ktW[1][i] += gw * phiT_coef<1, bb>(p) * D;
@@ -767,6 +755,24 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
                 if constexpr (BETA) u[i] += p[bb] * d1;
             }
             ktphi[bb] += lap;
+            if constexpr (DG) {
+                // X_i = D_j phi_1 D_k phi_2 - D_j phi_2 D_k phi_1 + D_j phi_3 D_k phi_4
+                //       - D_j phi_4 D_k phi_3, (i, j, k) cyclic (em_point)
+                if constexpr (bb == 0 || bb == 2) {
+#pragma unroll
+                    for (int i = 0; i < 3; i++) dput((bb == 0 ? 11 : 14) + i, Dk[i]);
+                } else {
+                    const double Dp[3] = {dget(bb == 1 ? 11 : 14), dget(bb == 1 ? 12 : 15), dget(bb == 1 ? 13 : 16)};
+                    double Xn[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        const int j = (i + 1) % 3, k = (i + 2) % 3;
+                        Xn[i] = bb == 1 ? Dp[j] * Dk[k] - Dk[j] * Dp[k] : (dget(11 + i) + Dp[j] * Dk[k]) - Dk[j] * Dp[k];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 3; i++) dput(11 + i, Xn[i]);
+                }
+            }
         } else if constexpr (kind == 3) {
             // Y_i: Laplacian into kt_Y, d_j Y_i into B^Y (beta terms only)
             constexpr int i = pp - 20;
@@ -787,9 +793,11 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             }
             ktY[i] += lap;
             if constexpr (DG && pp == 22) {
-                dput(14, 0.5 * (dd(BYv[0]) * dd(BYv[0]) + dd(BYv[1]) * dd(BYv[1]) + dd(BYv[2]) * dd(BYv[2])));
+                dput(9, 0.5 * (dd(BYv[0]) * dd(BYv[0]) + dd(BYv[1]) * dd(BYv[1]) + dd(BYv[2]) * dd(BYv[2])));
                 const double cY = dd(c[CH_SIG]) - dget(5);
                 dput(5, cY * cY);
+#pragma unroll
+                for (int i = 0; i < 3; i++) dput(14 + i, dd(BYv[i]));
             }
             if constexpr (BETA && pp == 22) {
                 // B^Y complete: beta_Y terms (main_rk4.jl:354, 553-581)
@@ -837,12 +845,41 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 #pragma unroll
                 for (int aa = 0; aa < 3; aa++)
                     me += dd(Bw[aa][0]) * dd(Bw[aa][0]) + dd(Bw[aa][1]) * dd(Bw[aa][1]) + dd(Bw[aa][2]) * dd(Bw[aa][2]);
-                dput(15, 0.5 * me);
+                dput(10, 0.5 * me);
 #pragma unroll
                 for (int aa = 0; aa < 3; aa++) {
                     const double cc = dd(c[CH_GAM + aa]) - dget(6 + aa);
                     dput(6 + aa, cc * cc);
                 }
+                // the electromagnetic densities of em_point (main_rk4.jl:2303-2822,
+                // 3718-4250): B = -(F + c1 X), B_2 = -(F + c2 X), A
+                const double p1 = dd(p[0]), p2 = dd(p[1]), p3 = dd(p[2]), p4 = dd(p[3]);
+                const double mag = p1 * p1 + p2 * p2 + p3 * p3 + p4 * p4;
+                double nv[3] = {0.0, 0.0, 0.0};
+                if (mag != 0.0) {
+                    nv[0] = -2.0 * (p1 * p3 + p2 * p4) / mag;
+                    nv[1] = 2.0 * (p2 * p3 - p1 * p4) / mag;
+                    nv[2] = (-p1 * p1 - p2 * p2 + p3 * p3 + p4 * p4) / mag;
+                }
+                const double c1 = 4.0 * sw / (P.gw * P.vev * P.vev);
+                const double c2 = mag != 0.0 ? 4.0 * sw / (P.gw * mag * mag) : 0.0;
+                double bb2 = 0.0, b22 = 0.0, b2a = 0.0;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const double Xi = dget(11 + i);
+                    const double F = sw * (nv[0] * dd(Bw[0][i]) + nv[1] * dd(Bw[1][i]) + nv[2] * dd(Bw[2][i])) +
+                                     cw * dget(14 + i);
+                    const double Be = -(F + c1 * Xi);
+                    const double B2 = mag != 0.0 ? -(F + c2 * Xi) : 0.0;
+                    const double Ai = sw * (nv[0] * dd(W(0, i)) + nv[1] * dd(W(1, i)) + nv[2] * dd(W(2, i))) +
+                                      cw * dd(c[CH_Y + i]);
+                    bb2 += Be * Be;
+                    b22 += B2 * B2;
+                    b2a += B2 * Ai;
+                }
+                dput(11, bb2);
+                dput(12, b22);
+                dput(13, b2a);
             }
             if constexpr (pp == 31) {
                 // B^a complete: -gw sum_j (W_j x F_ij)^a, beta_W terms
@@ -1119,20 +1156,56 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
         for (int m = 0; m < NK; m++) stn(b.ynew + (long)(16 + m) * b.n3, qb, o[m] * d6 + nstash[m * B::NT + t]);
     }
     if constexpr (DG) {
-        // the brick's partial sums: per-thread slots wave-reduced, then the
-        // waves combined in a fixed order
-        constexpr int slot_of[NKD] = {10, 11, 12, 13, 2, 17, 14, 15, 16, 0, 1, 7, 3, 5, 6, 4};
+        // the brick's partial sums: per-thread values wave-reduced, then the
+        // waves combined in a fixed order.  The densities of centre values
+        // alone, from the centres (the same expressions as the slots had).
+        double cen[5];   // PE, KE, |phi|, EEW, EEY
+        if constexpr (MIXED) {
+            const double r2d = dd(p[0]) * dd(p[0]) + dd(p[1]) * dd(p[1]) + dd(p[2]) * dd(p[2]) + dd(p[3]) * dd(p[3]);
+            const double X = r2d - P.vev * P.vev;
+            cen[0] = P.lam * (X * X);
+            cen[1] = dd(c[CH_PI]) * dd(c[CH_PI]) + dd(c[CH_PI + 1]) * dd(c[CH_PI + 1]) +
+                     dd(c[CH_PI + 2]) * dd(c[CH_PI + 2]) + dd(c[CH_PI + 3]) * dd(c[CH_PI + 3]);
+            cen[2] = sqrt(r2d);
+            double ee = 0.0;
+#pragma unroll
+            for (int qq = 0; qq < 9; qq++) ee += dd(c[CH_E + qq]) * dd(c[CH_E + qq]);
+            cen[3] = 0.5 * ee;
+            cen[4] = 0.5 * (dd(c[CH_EY]) * dd(c[CH_EY]) + dd(c[CH_EY + 1]) * dd(c[CH_EY + 1]) +
+                            dd(c[CH_EY + 2]) * dd(c[CH_EY + 2]));
+        } else {
+            const R X = r2 - R(P.vev) * R(P.vev);
+            cen[0] = R(P.lam) * (X * X);
+            cen[1] = c[CH_PI] * c[CH_PI] + c[CH_PI + 1] * c[CH_PI + 1] + c[CH_PI + 2] * c[CH_PI + 2] +
+                     c[CH_PI + 3] * c[CH_PI + 3];
+            cen[2] = sqrt(r2);
+            R ee = R(0);
+#pragma unroll
+            for (int qq = 0; qq < 9; qq++) ee += c[CH_E + qq] * c[CH_E + qq];
+            cen[3] = R(0.5) * ee;
+            cen[4] = R(0.5) * (c[CH_EY] * c[CH_EY] + c[CH_EY + 1] * c[CH_EY + 1] + c[CH_EY + 2] * c[CH_EY + 2]);
+        }
+        constexpr int cen_slot[5] = {0, 1, 7, 3, 5};
+        // slots 0..13 (14-16 are scratch)
+        constexpr int slot_of[14] = {10, 11, 12, 13, 2, 17, 14, 15, 16, 6, 4, 18, 19, 20};
         int bad = 0;
 #pragma unroll
-        for (int k = 0; k < NKD; k++) {
-            const double v = dget(k);
-            if (slot_of[k] < 8) bad += !isfinite(v);
+        for (int k = 0; k < 5; k++) {
+            const double v = cen[k];
+            bad += !isfinite(v);
             const double r = wave_reduce(v, false);
-            if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + slot_of[k]] = r;
-            if (k == 11) {
+            if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + cen_slot[k]] = r;
+            if (k == 2) {
                 const double rm = wave_reduce(v, true);
                 if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + 8] = rm;
             }
+        }
+#pragma unroll
+        for (int k = 0; k < 14; k++) {
+            const double v = dget(k);
+            if (slot_of[k] < 8 || slot_of[k] >= 18) bad += !isfinite(v);
+            const double r = wave_reduce(v, false);
+            if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + slot_of[k]] = r;
         }
         const double rb = wave_reduce(double(bad), false);
         if ((t & 63) == 0) dgw[(t >> 6) * NDIAG + 9] = rb;
@@ -1168,7 +1241,7 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 // 512-thread workgroup per CU).
 template <int TX, int TY, int TZ, int PF, bool BETA, typename T = double>
 __global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(waves_per_simd(false, PF, 1))))
-stage1_diag(StageBufs<T> b, BrickGeo g, Phys P, T dt, double *diag)
+stage1_diag(StageBufs<T> b, BrickGeo g, Phys P, T dt, double *diag, double sw, double cw)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     int bid = blockIdx.x;
@@ -1176,7 +1249,7 @@ stage1_diag(StageBufs<T> b, BrickGeo g, Phys P, T dt, double *diag)
         const int per = (g.nbx * g.nby * g.nbz) >> 3;
         bid = (bid & 7) * per + (bid >> 3);
     }
-    brick_body<TX, TY, TZ, PF, BETA, 1, T, true>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag);
+    brick_body<TX, TY, TZ, PF, BETA, 1, T, true>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag, sw, cw);
 }
 
 template <int TX, int TY, int TZ>
@@ -1225,11 +1298,11 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
 }
 
 template <int TX, int TY, int TZ, int PF, bool BETA, typename T>
-hipError_t launch_diag_beta(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, double *diag,
-                            hipStream_t st)
+hipError_t launch_diag_beta(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, double *diag, double sw,
+                            double cw, hipStream_t st)
 {
     using B = BrickShape<TX, TY, TZ>;
-    const size_t lds = 2 * B::BUF * sizeof(T) + ((B::NT / 64) * NDIAG + 16 * B::NT) * sizeof(double);
+    const size_t lds = 2 * B::BUF * sizeof(T) + ((B::NT / 64) * NDIAG + 17 * B::NT) * sizeof(double);
     static std::mutex attr_mu;
     static bool attr_set = false;
     {
@@ -1241,7 +1314,8 @@ hipError_t launch_diag_beta(const StageBufs<T> &b, const BrickGeo &g, const Phys
             attr_set = true;
         }
     }
-    stage1_diag<TX, TY, TZ, PF, BETA, T><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag);
+    stage1_diag<TX, TY, TZ, PF, BETA, T><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag, sw,
+                                                                                                cw);
     return hipGetLastError();
 }
 
@@ -1259,14 +1333,15 @@ BrickGeo make_brick_geo(int TX, int TY, int TZ, const ewpt_lattice &L, int xcd_m
     return g;
 }
 
-// stage 1 with the diagnostics (diag: one NDIAG row per brick)
+// stage 1 with the diagnostics (diag: one NDIAG row per brick; sw, cw: the
+// Weinberg angle of the EM densities)
 template <int TX, int TY, int TZ, int PF, typename T = double>
 hipError_t launch_tile_diag(const StageBufs<T> &b, const ewpt_lattice &L, const Phys &P, double dt, int xcd_map,
-                            int order, double *diag, hipStream_t st)
+                            int order, double *diag, double sw, double cw, hipStream_t st)
 {
     const BrickGeo g = make_brick_geo(TX, TY, TZ, L, xcd_map, order, b.zghost);
-    if (P.bw != 0.0 || P.by != 0.0) return launch_diag_beta<TX, TY, TZ, PF, true, T>(b, g, P, T(dt), diag, st);
-    return launch_diag_beta<TX, TY, TZ, PF, false, T>(b, g, P, T(dt), diag, st);
+    if (P.bw != 0.0 || P.by != 0.0) return launch_diag_beta<TX, TY, TZ, PF, true, T>(b, g, P, T(dt), diag, sw, cw, st);
+    return launch_diag_beta<TX, TY, TZ, PF, false, T>(b, g, P, T(dt), diag, sw, cw, st);
 }
 
 template <int TX, int TY, int TZ, int PF, typename T = double>

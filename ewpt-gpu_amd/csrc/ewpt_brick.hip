@@ -38,7 +38,7 @@ EWPT_DECLARE_BRICK(32)
 EWPT_DECLARE_BRICK(34)
 #undef EWPT_DECLARE_BRICK
 hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
-                                 int order, double *diag, hipStream_t st);
+                                 int order, double *diag, double sw, double cw, hipStream_t st);
 
 namespace {
 
@@ -93,10 +93,10 @@ bool brick_has_diag(int variant)
 }
 
 hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
-                              double *diag, hipStream_t st)
+                              double *diag, double sw, double cw, hipStream_t st)
 {
     if (!brick_has_diag(variant)) return hipErrorInvalidValue;
-    return launch_brick_v32_diag(b, L, P, dt, variant / 100, diag, st);
+    return launch_brick_v32_diag(b, L, P, dt, variant / 100, diag, sw, cw, st);
 }
 
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,

@@ -68,10 +68,10 @@ bool brick_has_diag_f32(int variant)
 long diag_rows_f32(const ewpt_lattice &L) { return (long)(L.nx / 32) * (L.ny / 4) * (L.nz / 4); }
 
 hipError_t launch_stage1_diag_f32(int variant, const StageBufs<float> &b, const ewpt_lattice &L, const Phys &P,
-                                  double dt, double *diag, hipStream_t st)
+                                  double dt, double *diag, double sw, double cw, hipStream_t st)
 {
     if (!brick_has_diag_f32(variant) || !brick_supported_f32(variant, L)) return hipErrorInvalidValue;
-    return launch_tile_diag<32, 4, 4, 1010704443, float>(b, L, P, dt, 1, variant / 100, diag, st);
+    return launch_tile_diag<32, 4, 4, 1010704443, float>(b, L, P, dt, 1, variant / 100, diag, sw, cw, st);
 }
 
 }  // namespace ewpt

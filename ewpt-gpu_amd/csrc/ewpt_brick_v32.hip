@@ -5,6 +5,8 @@
 // (fused_fields): each momentum and k value is read once.
 #include "ewpt_brick.h"
 
+static_assert(ewpt::NDIAG == ewpt::DIAG_SLOTS, "diagnostic row layout");
+
 namespace ewpt {
 
 bool brick_fits_v32(const ewpt_lattice &L) { return brick_fits<16, 4, 4>(L); }
@@ -16,9 +18,9 @@ hipError_t launch_brick_v32(int stage, const StageBufs<double> &b, const ewpt_la
 }
 
 hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
-                                 int order, double *diag, hipStream_t st)
+                                 int order, double *diag, double sw, double cw, hipStream_t st)
 {
-    return launch_tile_diag<16, 4, 4, 211704432>(b, L, P, dt, 1, order, diag, st);
+    return launch_tile_diag<16, 4, 4, 211704432>(b, L, P, dt, 1, order, diag, sw, cw, st);
 }
 
 }  // namespace ewpt

@@ -57,15 +57,17 @@ bool brick_stored_inputs(int variant);
 bool brick_supported(int variant, const ewpt_lattice &L);
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,
                               const Phys &P, double dt, hipStream_t st);
-// stage 1 that also reduces the energy densities and Gauss residuals of its
-// input (the state): one row of DIAG_SLOTS partial sums per brick in diag
-// (slots: PE KE GE EEW MEW EEY MEY sum|phi| min|phi| non-finite, then
-// (G^a)^2 (G^Y)^2 (C^a)^2 (C^Y)^2).  FP64 variants 32, 34 (+100*order).
-constexpr int DIAG_SLOTS = 18;
+// stage 1 that also reduces the energy densities, Gauss residuals and EM
+// densities of its input (the state): one row of DIAG_SLOTS partial sums per
+// brick in diag (slots: PE KE GE EEW MEW EEY MEY sum|phi| min|phi|
+// non-finite, then (G^a)^2 (G^Y)^2 (C^a)^2 (C^Y)^2, then |B|^2 |B_2|^2
+// B_2.A with sw, cw the sine and cosine of the Weinberg angle).  FP64
+// variants 32, 34 (+100*order).
+constexpr int DIAG_SLOTS = 21;
 bool brick_has_diag(int variant);
 long diag_rows(const ewpt_lattice &L);   // rows of diag (bricks of the diagnostic stage 1)
 hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
-                              double *diag, hipStream_t st);
+                              double *diag, double sw, double cw, hipStream_t st);
 // sum (slot 8: min) nblk rows of a row-major [nblk][nslot] array, slot by slot in a fixed order
 hipError_t launch_finish_rows(const double *part, long nblk, int nslot, unsigned minmask, double *res, hipStream_t st);
 
@@ -79,7 +81,7 @@ hipError_t launch_stage_brick_f32(int variant, int stage, const StageBufs<float>
 bool brick_has_diag_f32(int variant);
 long diag_rows_f32(const ewpt_lattice &L);
 hipError_t launch_stage1_diag_f32(int variant, const StageBufs<float> &b, const ewpt_lattice &L, const Phys &P,
-                                  double dt, double *diag, hipStream_t st);
+                                  double dt, double *diag, double sw, double cw, hipStream_t st);
 
 // ewpt_mixed.hip
 hipError_t launch_split(const double *y, float *hi, float *lo, long n, hipStream_t st);

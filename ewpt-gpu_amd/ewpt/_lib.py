@@ -42,7 +42,7 @@ EXPORTS = (
     "ewpt_slab_pack", "ewpt_slab_unpack", "ewpt_stage", "ewpt_get_stream", "ewpt_sync", "ewpt_slab_step",
     "ewpt_slab_step_timed", "ewpt_set_state_device", "ewpt_get_state_device",
     "ewpt_energies_device", "ewpt_init_thermal_gauss", "ewpt_step_diag", "ewpt_get_state_pair",
-    "ewpt_set_state_pair",
+    "ewpt_set_state_pair", "ewpt_step_diag_em",
 )
 
 
@@ -94,6 +94,7 @@ def load() -> ctypes.CDLL:
         "ewpt_step": [vp, pp, ip],
         "ewpt_step_timed": [vp, pp, ip, dp, dp],
         "ewpt_step_diag": [vp, pp, ip, dp, dp],
+        "ewpt_step_diag_em": [vp, pp, ctypes.c_double, ip, dp, dp],
         "ewpt_energies": [vp, pp, dp],
         "ewpt_gauss": [vp, pp, dp],
         "ewpt_set_variant": [vp, ip],

@@ -184,6 +184,15 @@ class Lattice:
                     "ewpt_step_diag")
         return e, g
 
+    def step_diag_em(self, p, theta_w: float, nsteps: int = 1):
+        """step_diag with the EM columns 7, 8, 12 of the energy row as well
+        (em_energies of the state before the steps, Weinberg angle theta_w)."""
+        P = as_params(p)
+        e, g = np.zeros(13), np.zeros(8)
+        self._check(self._L.ewpt_step_diag_em(self._h, ctypes.byref(P), float(theta_w), int(nsteps), _dptr(e),
+                                              _dptr(g)), "ewpt_step_diag_em")
+        return e, g
+
     def step_timed(self, p, nsteps: int):
         """Like step(); returns (mean ms per launch of stages 1..4, total ms)."""
         P = as_params(p)

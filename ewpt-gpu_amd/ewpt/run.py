@@ -69,6 +69,7 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
     try:
         p0 = P.step_params()
         cols = [0, 1, 2, 3, 4, 5, 6, 9, 10, 11]
+        cols_all = cols + [7, 8, 12]
         if resume:
             ck = checkpoint.load(checkpoint_path)
             if not np.array_equal(np.asarray(ck["run_ident"]), ident):
@@ -124,17 +125,17 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
             gam = damp(it, dt, mH, P.t_sat, P.gamma_inp, thermal_init=thermal_init)
             p = P.step_params(jex=jex, gamma=gam)
             if pending is not None:
-                # the energy table of the snapshot state, taken by this step's
-                # stage 1 from the values it stages anyway (ewpt_step_diag)
-                e, _ = lat.step_diag(p, 1)
-                energies[pending[0], cols] = e[cols]
+                # the energy row of the snapshot state (EM columns included),
+                # taken by this step's stage 1 from the values it stages
+                # anyway (ewpt_step_diag_em)
+                e, _ = lat.step_diag_em(p, theta_w, 1)
+                energies[pending[0], cols_all] = e[cols_all]
                 report(*pending)
                 pending = None
             else:
                 lat.step(p, 1)
             if it % dsnaps == 0 and snp < P.nsnaps:
                 snp += 1
-                energies[snp, [7, 8, 12]] = lat.em_energies(p, theta_w)
                 slices[snp] = np.transpose(lat.y_slice(p, theta_w), (0, 2, 1))
                 pending = (snp, it)
             if out_dir and write_fft and it % dsnaps_fft == 0:
@@ -143,6 +144,7 @@ def run(P: Parameters, thermal_init: bool = True, profile="ref", precision: int 
                 save_checkpoint(it)
         if pending is not None:
             energies[pending[0], cols] = lat.energies(P.step_params())[cols]
+            energies[pending[0], [7, 8, 12]] = lat.em_energies(P.step_params(), theta_w)
             report(*pending)
         k1, s1 = lat.spectrum(P.step_params(), theta_w)
         res.update(energies=energies, slices=slices, spectrum=np.stack([np.stack([k0, s0], 1), np.stack([k1, s1], 1)]),

@@ -180,7 +180,8 @@ int ewpt_rhs_device(const ewpt_lattice *lat, const ewpt_params *p,
 int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
 
 /* nsteps RK4 steps as ewpt_step, and the energy table (ewpt_energies
- * layout: columns 0-6, 9, 11; the EM columns 7, 8, 12 are 0) and the Gauss
+ * layout: columns 0-6, 9, 11; the EM columns 7, 8, 12 are 0 here and filled
+ * by ewpt_step_diag_em below) and the Gauss
  * norms (ewpt_gauss layout) of the state before the first step.  On the
  * FP64 path with stage variant 32 or 34 (34 is the automatic one where it
  * tiles) the first step's stage 1 reduces them from the values it stages anyway (the
@@ -196,6 +197,13 @@ int ewpt_step(ewpt_ctx *ctx, const ewpt_params *p, int nsteps);
  * for the step after the snapshot.  Returns EWPT_ERR_NONFINITE (after
  * stepping and filling the outputs) if a density is NaN/Inf. */
 int ewpt_step_diag(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, double energies[13], double gauss[8]);
+/* ewpt_step_diag with the EM columns 7, 8, 12 of the energy table as well
+ * (ewpt_em_energies of the same state with Weinberg angle theta_w: the
+ * magnetic energies of compute_B_* / compute_B_2_*, main_rk4.jl:2303-2822,
+ * 3718-4250, as the snapshot block takes them, 6861-6901), reduced by the
+ * same stage 1: every column of a snapshot's energy row from one pass. */
+int ewpt_step_diag_em(ewpt_ctx *ctx, const ewpt_params *p, double theta_w, int nsteps, double energies[13],
+                      double gauss[8]);
 
 /* Like ewpt_step, and reports mean device time per launch (ms) of each
  * stage kernel, measured with HIP events on the context stream:

@@ -5,15 +5,18 @@ Tolerances: single RHS evaluations agree to 1e-12 relative L2 per channel
 with fused multiply-adds); evolved states to 1e-11 relative L2 per channel,
 the bound the FP64 path is held to after 200 RK4 steps.
 """
+import math
+
 import numpy as np
 import pytest
 
 import ewpt
-from oracle import cref, fields
+from oracle import cref, fields, spec_np
 
 pytestmark = pytest.mark.gpu
 
 DX = 0.25
+TH = math.asin(math.sqrt(0.22))   # the Weinberg angle of the EM columns
 
 
 def rel_l2_per_channel(a, b):
@@ -300,7 +303,8 @@ def test_step_diag_matches_standalone_and_oracle(precision, shape):
     """ewpt_step_diag: the energy table and Gauss norms of the state before the
     step, reduced by the FP64 stage 1 on the way (variants 32 and 34), equal the
     standalone reductions and the oracle to 1e-12; the step itself equals a
-    plain step to a few roundings."""
+    plain step to a few roundings.  ewpt_step_diag_em adds the EM columns
+    (ewpt_em_energies and the oracle to 1e-12)."""
     p = fields.params(gamma=0.1, jex=0.2)
     s = fields.smooth_state(shape, seed=31, amp=0.4)
     with ewpt.Lattice(shape, DX, precision=precision) as a, ewpt.Lattice(shape, DX, precision=precision) as b:
@@ -318,6 +322,14 @@ def test_step_diag_matches_standalone_and_oracle(precision, shape):
         assert np.abs(ya - yb).max() <= 64 * np.finfo(np.float64).eps * np.abs(yb).max()
         e2, g2 = a.step_diag(p, 1)                 # the state after two steps
         e2_want, g2_want = b.energies(p), b.gauss(p)
+        # with the EM columns: the same stage 1, other columns bit for bit
+        b.set_state(s)
+        m_want = b.em_energies(p, TH)
+        em, gm = b.step_diag_em(p, TH, 1)
+    np.testing.assert_allclose(em[[7, 8, 12]], m_want, rtol=1e-12)
+    np.testing.assert_allclose(em[[7, 8, 12]], spec_np.em_energies(cref.em_fields(s, p, DX, TH), DX), rtol=1e-12)
+    keep = [0, 1, 2, 3, 4, 5, 6, 9, 10, 11]
+    assert np.array_equal(em[keep], e[keep]) and np.array_equal(gm, g)
     np.testing.assert_allclose(e, e_want, rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(g, g_want, rtol=1e-11, atol=1e-15)
     np.testing.assert_allclose(e2, e2_want, rtol=1e-12, atol=1e-15)
@@ -330,9 +342,9 @@ def test_step_diag_matches_standalone_and_oracle(precision, shape):
 
 @pytest.mark.parametrize("shape", [(64, 32, 32), (32, 8, 16)])
 def test_step_diag_mixed(shape):
-    """Mixed path: stage 1 of variants 31/33 reduces the energy table and Gauss
-    norms of its input in FP64 arithmetic from the FP32 high words and the FP32
-    pass derivatives.  They agree with the FP64 reductions of the state (high
+    """Mixed path: stage 1 of variants 31/33 reduces the energy table (EM
+    columns included) and Gauss norms of its input in FP64 arithmetic from the
+    FP32 high words and the FP32 pass derivatives.  They agree with the FP64 reductions of the state (high
     + compensation word) and with the oracle to the FP32 rounding of the
     fields (2e-5 relative; measured at ~1e-6), and the step equals a plain
     mixed step to a few FP32 roundings."""
@@ -352,6 +364,14 @@ def test_step_diag_mixed(shape):
         assert np.abs(ya - yb).max() <= 64 * eps32 * np.abs(yb).max()
         e2, g2 = a.step_diag(p, 1)                 # the state after two steps
         e2_want, g2_want = b.energies(p), b.gauss(p)
+        b.set_state(s)
+        m_want = b.em_energies(p, TH)
+        em, _ = b.step_diag_em(p, TH, 1)
+    m_ref = spec_np.em_energies(cref.em_fields(s, p, DX, TH), DX)
+    for got, want in ((em[[7, 8, 12]], m_want), (em[[7, 8, 12]], m_ref)):
+        rel = np.abs(got - want) / np.abs(want)
+        print("em rel", np.array2string(rel, precision=2))
+        assert rel.max() < 2e-5, rel
     er, gr = cref.energies(s, p, DX), cref.gauss(s, p, DX)
     for got, want in ((e, e_want), (e2, e2_want), (e, er)):
         rel = np.abs(got[cols] - want[cols]) / np.abs(want[cols])
