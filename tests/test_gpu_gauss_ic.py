@@ -9,10 +9,10 @@ k_Gamma / k_Sigma charges of main_rk4.jl:603-657) and of the library
   the oracle's to 1e-12 relative L2 per channel, and the Gauss residual
   after the projection is below 1e-12 of the one before it.
 * The thermal configuration's size, 256^3: the residual drops below 1e-12 of
-  its start; a 200-step FP64 evolution from it grows the residual only at
-  the level the finite-difference equations produce (printed beside the
-  growth from the reference's pi = E = 0 start), far below the residual of
-  the unprojected draw.
+  its start; over the configuration's 500 FP64 steps the residual grows to
+  the level the equations produce from the reference's own pi = E = 0 start
+  (within 2x of it), far below the residual of the unprojected draw (below
+  1/10 of it after 200 steps, 1/5 after 500).
 """
 import numpy as np
 import pytest
@@ -80,16 +80,24 @@ def test_gauss_ic_config_256():
         print(f"256^3 solve: {st}", flush=True)
         assert st["residual_after"] < 1e-12 * st["residual_before"], st
         lat.step(p, 200)
+        g200 = lat.gauss(p)
+        lat.step(p, 300)
         g1 = lat.gauss(p)
         e1 = lat.energies(p)
     with ewpt.Lattice(n, DX) as lat:
         lat.init_thermal(T, P.meff_sq, P.base_seed, "config")   # pi = E = 0
         lat.step(p, 200)
+        gz200 = lat.gauss(p)
+        lat.step(p, 300)
         g_zero = lat.gauss(p)
-    G0, G1, Gz = (float(np.sqrt(np.sum(np.asarray(g[:4]) ** 2))) for g in (g0, g1, g_zero))
+    G0, G200, G1, Gz200, Gz = (float(np.sqrt(np.sum(np.asarray(g[:4]) ** 2))) for g in (g0, g200, g1, gz200, g_zero))
     drift = (sum(e1[:7]) - sum(e0[:7])) / sum(e0[:7])
-    print(f"Gauss residual: start {G0:.3e}, after 200 steps {G1:.3e} (from pi = E = 0: {Gz:.3e}); "
-          f"energy drift {drift:.3e}", flush=True)
+    print(f"Gauss residual: start {G0:.3e}, after 200 / 500 steps {G200:.3e} / {G1:.3e} "
+          f"(from pi = E = 0: {Gz200:.3e} / {Gz:.3e}); energy drift {drift:.3e}", flush=True)
     assert G0 < 1e-12 * st["residual_before"]
-    assert G1 < 0.1 * st["residual_before"]
+    # the residual the equations build up (the k_Gamma / k_Sigma terms do not
+    # keep div E = rho) is that of the reference's own pi = E = 0 start, far
+    # below the unprojected draw's
+    assert G200 < 0.1 * st["residual_before"]
+    assert G1 < 2.0 * Gz and G1 < 0.2 * st["residual_before"]
     assert abs(drift) < 1e-3
