@@ -333,6 +333,13 @@ __host__ __device__ constexpr bool fused_fields(int PF) { return (PF / 100000000
 // u2 and k1 (20 channels) instead of y (36).  Each reconstruction rounds once
 // more than the reference's y + c k (about one unit in the last place).
 __host__ __device__ constexpr bool recon_y(int PF) { return (PF / 100000000) % 10 == 3; }
+// Channel 16+m whose stage-4 update the spread passes form in pass q: with
+// recon_y the order in which the passes stage the centres y is rebuilt from
+// (E, Gamma/Sigma, then pi), else m = q.
+__host__ __device__ constexpr int upd_channel(int PF, int q)
+{
+    return !recon_y(PF) ? q : q < 16 ? pass_channel(q) - 16 : 4 + (q - 16);
+}
 __host__ __device__ constexpr int waves_per_simd(bool f32, int PF, int S) { return f32 && (S < 4 || lean_stash(PF)) ? 4 : 2; }
 
 // T: storage and arithmetic type (double, or float for the mixed path).
@@ -988,93 +995,119 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             (body(std::integral_constant<int, I>{}), ...);
         }(std::make_integer_sequence<int, NPASS>{});
     } else if constexpr (!paired_of(PF, S)) {
-        // stage 4 with the partial sums spread over the first NK passes
-        store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
-        lds_barrier();
-        auto body = [&](auto pc) {
-            constexpr int pp = decltype(pc)::value;
-            if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
-            if constexpr (pp < NK && MIXED) {
-                // the increment of channel 16+pp (and with pp >= 4 the field
-                // channel pp - 4, whose momentum it is), as the prologue forms
-                // them; the loads are in flight while the pass computes
-                const R k1v = ldo(K(b.k1, pp), qb), k2v = ldo(K(b.k2, pp), qb);
-                const R k3v = ldo(K(b.k3, pp), qb), ylv = ldo(b.ylo + (long)(16 + pp) * b.n3, qb);
-                consume(pc, lds[pp & 1]);
-                pin_all(pc);
-                R dy = k1v * d6;
-                dy = k2v * d3 + dy;
-                dy = k3v * d3 + dy;
-                nstash[pp * B::NT + t] = ylv + dy;
-                if constexpr (!lean_stash(PF)) nstash[(NK + pp) * B::NT + t] = ldo(Y(16 + pp), qb);
-                if constexpr (fused_fields(PF) && pp >= 4) {
-                    constexpr int ch = pp - 4;
-                    const R piv = ldo(Y(ch + 20), qb);
-                    const R k2 = piv + k1v * half_dt;
-                    const R k3 = piv + k2v * half_dt;
-                    const R k4 = piv + k3v * dt;
+        // stage 4 with the partial sums spread over the first NK passes.  The
+        // inputs of pass q's update (channel 16+m with its k's, and with
+        // fused_fields the field channel whose momentum it is) are loaded one
+        // pass earlier, before that pass's halo fetch: a use of a loaded value
+        // waits for every load issued before it, so inputs loaded after the
+        // fetch of pass q + PFS and used in pass q made the pass wait for that
+        // halo too (512^3: FP64 334 stage 4 37.0 -> 33.2 ms, mixed 333 28.9 ->
+        // 26.1 ms; loading them in their own pass before the fetch gave 37.0 /
+        // 26.3 ms, two passes ahead 32.9 / 28.5 ms, the mixed kernel spilling).
+        constexpr int UA = 1;
+        struct Upd {
+            R k1, k2, k3;   // k1..k3 of channel 16+m
+            R y, lo;        // y (mixed: high and compensation word) of channel 16+m
+            R f, lf;        // y of field channel m-4 (and its compensation word)
+        };
+        auto upd_load = [&](auto qc, Upd &u) {
+            constexpr int m = upd_channel(PF, decltype(qc)::value);
+            u.k1 = ldo(K(b.k1, m), qb);
+            u.k2 = ldo(K(b.k2, m), qb);
+            u.k3 = ldo(K(b.k3, m), qb);
+            if constexpr (MIXED) {
+                u.lo = ldo(b.ylo + (long)(16 + m) * b.n3, qb);
+                if constexpr (!lean_stash(PF) || (fused_fields(PF) && m >= 4)) u.y = ldo(Y(16 + m), qb);
+                if constexpr (fused_fields(PF) && m >= 4) {
+                    u.f = ldo(Y(m - 4), qb);
+                    u.lf = ldo(b.ylo + (long)(m - 4) * b.n3, qb);
+                }
+            } else if constexpr (!recon_y(PF)) {
+                u.y = ldo(Y(16 + m), qb);
+                if constexpr (fused_fields(PF) && m >= 4) u.f = ldo(Y(m - 4), qb);
+            }
+        };
+        auto upd_apply = [&](auto qc, const Upd &u) {
+            constexpr int m = upd_channel(PF, decltype(qc)::value);
+            if constexpr (MIXED) {
+                // the increment of channel 16+m (and with m >= 4 the field
+                // channel m - 4, whose momentum it is), as the prologue forms them
+                R dy = u.k1 * d6;
+                dy = u.k2 * d3 + dy;
+                dy = u.k3 * d3 + dy;
+                nstash[m * B::NT + t] = u.lo + dy;
+                if constexpr (!lean_stash(PF)) nstash[(NK + m) * B::NT + t] = u.y;
+                if constexpr (fused_fields(PF) && m >= 4) {
+                    constexpr int ch = m - 4;
+                    const R piv = u.y;
+                    const R k2 = piv + u.k1 * half_dt;
+                    const R k3 = piv + u.k2 * half_dt;
+                    const R k4 = piv + u.k3 * dt;
                     R dyf = piv * d6;
                     dyf = k2 * d3 + dyf;
                     dyf = k3 * d3 + dyf;
                     dyf = k4 * d6 + dyf;
                     R nh, nl;
-                    two_sum(ldo(Y(ch), qb), ldo(b.ylo + (long)ch * b.n3, qb) + dyf, nh, nl);
+                    two_sum(u.f, u.lf + dyf, nh, nl);
                     stn(b.ynew + (long)ch * b.n3, qb, nh);
                     stn(b.ylo_new + (long)ch * b.n3, qb, nl);
                 }
-            } else if constexpr (pp < NK && recon_y(PF)) {
-                // the partial sum of channel 16+m, m in the order the passes set
-                // the centres of the stage input (E, Gamma/Sigma, pi), with y
-                // reconstructed from that centre: u4 = y + k3 dt
-                constexpr int m = pp < 16 ? pass_channel(pp) - 16 : 4 + (pp - 16);
+            } else if constexpr (recon_y(PF)) {
+                // the partial sum of channel 16+m with y reconstructed from the
+                // centre of the stage input this pass has set: u4 = y + k3 dt
                 static_assert(stored_in(PF, 4), "y is reconstructed from the stored stage input");
-                const R k1v = ldo(K(b.k1, m), qb), k2v = ldo(K(b.k2, m), qb), k3v = ldo(K(b.k3, m), qb);
-                consume(pc, lds[pp & 1]);
-                pin_all(pc);
-                const R yv = c[16 + m] - k3v * dt;
-                R n = k1v * d6 + yv;
-                n = k2v * d3 + n;
-                nstash[m * B::NT + t] = k3v * d3 + n;
+                const R yv = c[16 + m] - u.k3 * dt;
+                R n = u.k1 * d6 + yv;
+                n = u.k2 * d3 + n;
+                nstash[m * B::NT + t] = u.k3 * d3 + n;
                 if constexpr (m >= 4) {
                     // field channel m - 4, whose momentum is channel 16 + m:
                     // u4_f = y_f + k3_f dt
                     constexpr int ch = m - 4;
                     const R piv = yv;
-                    const R k2 = piv + k1v * half_dt;
-                    const R k3 = piv + k2v * half_dt;
-                    const R k4 = piv + k3v * dt;
+                    const R k2 = piv + u.k1 * half_dt;
+                    const R k3 = piv + u.k2 * half_dt;
+                    const R k4 = piv + u.k3 * dt;
                     R nf = piv * d6 + (c[ch] - k3 * dt);
                     nf = k2 * d3 + nf;
                     nf = k3 * d3 + nf;
                     stn(b.ynew + (long)ch * b.n3, qb, k4 * d6 + nf);
                 }
-            } else if constexpr (pp < NK) {
-                // the partial sum of channel 16+pp: its loads are in flight while
-                // the pass computes
-                const R k1v = ldo(K(b.k1, pp), qb), k2v = ldo(K(b.k2, pp), qb);
-                const R k3v = ldo(K(b.k3, pp), qb), yv = ldo(Y(16 + pp), qb);
-                consume(pc, lds[pp & 1]);
-                pin_all(pc);
-                R n = k1v * d6 + yv;
-                n = k2v * d3 + n;
-                nstash[pp * B::NT + t] = k3v * d3 + n;
-                if constexpr (fused_fields(PF) && pp >= 4) {
-                    // field channel pp - 4: its momentum is channel 16 + pp,
+            } else {
+                R n = u.k1 * d6 + u.y;
+                n = u.k2 * d3 + n;
+                nstash[m * B::NT + t] = u.k3 * d3 + n;
+                if constexpr (fused_fields(PF) && m >= 4) {
+                    // field channel m - 4: its momentum is channel 16 + m,
                     // the expressions of the field loop
-                    constexpr int ch = pp - 4;
-                    const R piv = yv;
-                    const R k2 = piv + k1v * half_dt;
-                    const R k3 = piv + k2v * half_dt;
-                    const R k4 = piv + k3v * dt;
-                    R nf = piv * d6 + ldo(Y(ch), qb);
+                    constexpr int ch = m - 4;
+                    const R piv = u.y;
+                    const R k2 = piv + u.k1 * half_dt;
+                    const R k3 = piv + u.k2 * half_dt;
+                    const R k4 = piv + u.k3 * dt;
+                    R nf = piv * d6 + u.f;
                     nf = k2 * d3 + nf;
                     nf = k3 * d3 + nf;
                     stn(b.ynew + (long)ch * b.n3, qb, k4 * d6 + nf);
                 }
-            } else {
-                consume(pc, lds[pp & 1]);
-                pin_all(pc);
             }
+        };
+        Upd up[UA + 1];
+        [&]<int... I>(std::integer_sequence<int, I...>) {
+            (upd_load(std::integral_constant<int, I>{}, up[I]), ...);
+        }(std::make_integer_sequence<int, UA>{});
+        store(std::integral_constant<int, 0>{}, rr[0], lds[0]);
+        lds_barrier();
+        auto body = [&](auto pc) {
+            constexpr int pp = decltype(pc)::value;
+            if constexpr (pp + UA < NK) {
+                upd_load(std::integral_constant<int, pp + UA>{}, up[(pp + UA) % (UA + 1)]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
+            consume(pc, lds[pp & 1]);
+            pin_all(pc);
+            if constexpr (pp < NK) upd_apply(pc, up[pp % (UA + 1)]);
             if constexpr (pp + 1 < NPASS)
                 store(std::integral_constant<int, pp + 1>{}, rr[(pp + 1) % PFS], lds[(pp + 1) & 1]);
             lds_barrier();
