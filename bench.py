@@ -84,19 +84,19 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # was measured; a kernel changed since then needs a new counter run.
 TRAFFIC_SOURCE = {
     "r1": "profiles/stage_traffic_512.md (library at commit be83b44 / 1ea9c7a)",
-    "r2": "profiles/stage_traffic_512.md (library at commit ab268f1)",
+    "r3": "profiles/stage_traffic_512.md (library at commit 246ae65)",
 }
 MEASURED_TRAFFIC = {
-    ("fp64", 334, 1): (1114, "r2"), ("fp64", 334, 2): (1335, "r2"), ("fp64", 334, 3): (1474, "r2"),
-    ("fp64", 334, 4): (1544, "r2"),
+    ("fp64", 334, 1): (1113, "r3"), ("fp64", 334, 2): (1343, "r3"), ("fp64", 334, 3): (1474, "r3"),
+    ("fp64", 334, 4): (1554, "r3"),
     ("fp64", 332, 1): (1110, "r1"), ("fp64", 332, 2): (1504, "r1"), ("fp64", 332, 3): (1500, "r1"),
     ("fp64", 332, 4): (1882, "r1"),
     ("fp64", 324, 1): (1154, "r1"), ("fp64", 324, 2): (1387, "r1"), ("fp64", 324, 3): (1386, "r1"),
     ("fp64", 324, 4): (2407, "r1"),
     ("fp64", 311, 1): (849, "r1"), ("fp64", 311, 2): (1630, "r1"), ("fp64", 311, 3): (2212, "r1"),
     ("fp64", 311, 4): (3725, "r1"),
-    ("mixed", 333, 1): (597, "r2"), ("mixed", 333, 2): (716, "r2"), ("mixed", 333, 3): (712, "r2"),
-    ("mixed", 333, 4): (1324, "r2"),
+    ("mixed", 333, 1): (602, "r3"), ("mixed", 333, 2): (714, "r3"), ("mixed", 333, 3): (716, "r3"),
+    ("mixed", 333, 4): (1321, "r3"),
     ("mixed", 331, 1): (632, "r1"), ("mixed", 331, 2): (850, "r1"), ("mixed", 331, 3): (845, "r1"),
     ("mixed", 331, 4): (1495, "r1"),
     ("mixed", 315, 1): (628, "r1"), ("mixed", 315, 2): (696, "r1"), ("mixed", 315, 3): (698, "r1"),

@@ -208,9 +208,9 @@ int push_view(ewpt_ctx *c)
 }
 
 // variant actually launched: AUTO takes the first brick variant that tiles
-// the lattice from the measured ranking (512^3 on MI355X, FP64: 334 130.7,
+// the lattice from the measured ranking (512^3 on MI355X, FP64: 334 124.3,
 // 332 140.6, 324 170.3, 320 181.2, 311 184.4, 310 187.0 ms/step; mixed:
-// 333 78.0, 331 82.7, 315 89.4, 311 96.3, 211 103.2 ms/step), else the
+// 333 75.8, 331 82.7, 315 89.4, 311 96.3, 211 103.2 ms/step), else the
 // gather kernel.  The stored-stage-input variants (334, 333, ...) hold one
 // more state-sized buffer; without room for it AUTO skips them.
 int resolved_variant(const ewpt_ctx *c)
