@@ -887,7 +887,7 @@ int step_diag_impl(ewpt_ctx *ctx, const ewpt_params *p, int nsteps, bool em, dou
         HIPCHK(ctx, launch_stage1_diag(v, bufs_for(ctx, 1, brick_stored_inputs(v)), ctx->lat, phys_of(*p), p->dt,
                                        ctx->diag, sw, cw, ctx->stream),
                "diagnostic stage launch");
-    HIPCHK(ctx, launch_finish_rows(ctx->diag, rows, DIAG_SLOTS, 1u << 8, res, ctx->stream), "diagnostic reduction");
+    HIPCHK(ctx, launch_fixed_reduce(ctx->diag, rows, DIAG_SLOTS, DIAG_SLOTS, 1u << 8, res, ctx->stream), "diagnostic reduction");
     double r[DIAG_SLOTS];
     HIPCHK(ctx, hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, ctx->stream), "diagnostic copy");
     for (int s = 2; s <= 4; s++)

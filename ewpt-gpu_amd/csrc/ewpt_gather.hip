@@ -631,15 +631,46 @@ gauss_brick(const double *y, long n3, Geo g, Phys P, double *part)
     block_reduce<10>(v, 0u, part);
 }
 
-// Sum (or min) nblk partials of slot q in a fixed order; one block per slot.
-__global__ void __launch_bounds__(256) finish_reduce(const double *part, long nblk, unsigned minmask, double *res)
+// Fixed-order sum (or min, slots set in minmask) over nblk rows of partials,
+// `stride` doubles per row, slots 0..ns-1 used.  Two passes:
+// rows_partial: block g reduces the rows [g R, min((g+1) R, nblk)) of every
+// slot -- thread (q, j) = (t % ns, t / ns) takes rows j, j + per, ... of slot
+// q, so a sweep of the block reads per consecutive rows -- then combines the
+// per threads of each slot in order of j and writes the result over the
+// chunk's first row, which no other block reads; rows_final: block q reduces
+// slot q of those chunk rows.  The grouping depends on nblk only, so every
+// call on the same partials gives the same result.
+__global__ void __launch_bounds__(256) rows_partial(double *part, long nblk, int stride, int ns, long R,
+                                                    unsigned minmask)
+{
+    __shared__ double sh[256];
+    const int t = threadIdx.x, per = 256 / ns, q = t % ns, j = t / ns;
+    const long b0 = (long)blockIdx.x * R, b1 = b0 + R < nblk ? b0 + R : nblk;
+    const bool mn = (minmask >> q) & 1u;
+    double acc = mn ? INFINITY : 0.0;
+    if (j < per)
+        for (long b = b0 + j; b < b1; b += per) {
+            const double v = part[b * stride + q];
+            acc = mn ? fmin(acc, v) : acc + v;
+        }
+    sh[t] = acc;
+    __syncthreads();
+    if (t < ns) {
+        double a = sh[t];
+        for (int jj = 1; jj < per; jj++) a = mn ? fmin(a, sh[jj * ns + t]) : a + sh[jj * ns + t];
+        part[b0 * stride + t] = a;
+    }
+}
+
+__global__ void __launch_bounds__(256) rows_final(const double *part, int stride, long R, long G, unsigned minmask,
+                                                  double *res)
 {
     __shared__ double sh[256];
     const int q = blockIdx.x;
     const bool mn = (minmask >> q) & 1u;
     double acc = mn ? INFINITY : 0.0;
-    for (long b = threadIdx.x; b < nblk; b += 256) {
-        const double v = part[b * NRED + q];
+    for (long g = threadIdx.x; g < G; g += 256) {
+        const double v = part[g * R * stride + q];
         acc = mn ? fmin(acc, v) : acc + v;
     }
     sh[threadIdx.x] = acc;
@@ -709,12 +740,10 @@ hipError_t launch_energies(const double *y, const ewpt_lattice &L, const Phys &P
     if (energy_brick_fits(L)) {
         const long nb = n3 / EBT;   // <= reduce_blocks(L), the partials' capacity
         energy_brick<<<dim3((unsigned)nb), dim3(EBT), 0, st>>>(y, n3, g, P, part);
-        finish_reduce<<<10, 256, 0, st>>>(part, nb, 1u << 8, res);
-    } else {
-        energy_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
-        finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 1u << 8, res);
+        return launch_fixed_reduce(part, nb, NRED, 10, 1u << 8, res, st);
     }
-    return hipGetLastError();
+    energy_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
+    return launch_fixed_reduce(part, reduce_blocks(L), NRED, 10, 1u << 8, res, st);
 }
 
 hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, double *part, double *res,
@@ -725,41 +754,20 @@ hipError_t launch_gauss(const double *y, const ewpt_lattice &L, const Phys &P, d
     if (energy_brick_fits(L)) {
         const long nb = n3 / EBT;   // <= reduce_blocks(L), the partials' capacity
         gauss_brick<<<dim3((unsigned)nb), dim3(EBT), 0, st>>>(y, n3, g, P, part);
-        finish_reduce<<<10, 256, 0, st>>>(part, nb, 0u, res);
-    } else {
-        gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
-        finish_reduce<<<10, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+        return launch_fixed_reduce(part, nb, NRED, 10, 0u, res, st);
     }
-    return hipGetLastError();
+    gauss_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, part);
+    return launch_fixed_reduce(part, reduce_blocks(L), NRED, 10, 0u, res, st);
 }
 
-namespace {
-
-__global__ void __launch_bounds__(256) finish_rows(const double *part, long nblk, int nslot, unsigned minmask,
-                                                  double *res)
+hipError_t launch_fixed_reduce(double *part, long nblk, int stride, int ns, unsigned minmask, double *res,
+                               hipStream_t st)
 {
-    __shared__ double sh[256];
-    const int q = blockIdx.x;
-    const bool mn = (minmask >> q) & 1u;
-    double acc = mn ? INFINITY : 0.0;
-    for (long b = threadIdx.x; b < nblk; b += 256) {
-        const double v = part[b * nslot + q];
-        acc = mn ? fmin(acc, v) : acc + v;
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) sh[threadIdx.x] = mn ? fmin(sh[threadIdx.x], sh[threadIdx.x + st]) : sh[threadIdx.x] + sh[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) res[q] = sh[0];
-}
-
-}  // namespace
-
-hipError_t launch_finish_rows(const double *part, long nblk, int nslot, unsigned minmask, double *res, hipStream_t st)
-{
-    finish_rows<<<nslot, 256, 0, st>>>(part, nblk, nslot, minmask, res);
+    if (nblk < 1 || ns < 1 || ns > 32 || stride < ns) return hipErrorInvalidValue;
+    const long G0 = nblk < 1024 ? nblk : 1024;   // at most 1024 chunks of R rows
+    const long R = (nblk + G0 - 1) / G0, G = (nblk + R - 1) / R;
+    rows_partial<<<(unsigned)G, 256, 0, st>>>(part, nblk, stride, ns, R, minmask);
+    rows_final<<<ns, 256, 0, st>>>(part, stride, R, G, minmask, res);
     return hipGetLastError();
 }
 
@@ -790,12 +798,10 @@ hipError_t launch_em_energies(const double *y, const ewpt_lattice &L, const Phys
     if (energy_brick_fits(L)) {
         const long nb = n3 / EBT;   // <= reduce_blocks(L), the partials' capacity
         em_brick<<<dim3((unsigned)nb), dim3(EBT), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
-        finish_reduce<<<4, 256, 0, st>>>(part, nb, 0u, res);
-    } else {
-        em_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
-        finish_reduce<<<4, 256, 0, st>>>(part, reduce_blocks(L), 0u, res);
+        return launch_fixed_reduce(part, nb, NRED, 4, 0u, res, st);
     }
-    return hipGetLastError();
+    em_density<<<grid_of(L), dim3(BX, BY, 1), 0, st>>>(y, n3, g, P, sin(theta_w), cos(theta_w), part);
+    return launch_fixed_reduce(part, reduce_blocks(L), NRED, 4, 0u, res, st);
 }
 
 // ---- the allocation-free energy table (ewpt_energies_device)
@@ -848,7 +854,7 @@ hipError_t launch_energies_noalloc(const double *y, const ewpt_lattice &L, const
     const int grid = (int)std::min<long>(nb, NOALLOC_BLOCKS);
     double *part = scr, *res = scr + (size_t)NOALLOC_BLOCKS * NRED;
     energy_brick_strided<<<grid, EBT, 0, st>>>(y, n3, g, P, nb, part);
-    finish_reduce<<<10, 256, 0, st>>>(part, grid, 1u << 8, res);
+    if ((e = launch_fixed_reduce(part, grid, NRED, 10, 1u << 8, res, st)) != hipSuccess) return e;
     energy_table<<<1, 64, 0, st>>>(res, L.dx * L.dx * L.dx, (double)n3, out);
     return hipGetLastError();
 }

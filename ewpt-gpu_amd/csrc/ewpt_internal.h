@@ -69,7 +69,11 @@ long diag_rows(const ewpt_lattice &L);   // rows of diag (bricks of the diagnost
 hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                               double *diag, double sw, double cw, hipStream_t st);
 // sum (slot 8: min) nblk rows of a row-major [nblk][nslot] array, slot by slot in a fixed order
-hipError_t launch_finish_rows(const double *part, long nblk, int nslot, unsigned minmask, double *res, hipStream_t st);
+// Fixed-order sum / min (slots in minmask) of nblk rows of partials, stride
+// doubles per row, slots 0..ns-1 (ns <= 32) into res[0..ns-1]; overwrites
+// some rows of part.
+hipError_t launch_fixed_reduce(double *part, long nblk, int stride, int ns, unsigned minmask, double *res,
+                               hipStream_t st);
 
 // ewpt_brick_f32.hip -- mixed-precision stage kernels
 bool brick_exists_f32(int variant);
