@@ -374,7 +374,9 @@ __device__ __forceinline__ double wave_reduce(double v, bool mn)
     return v;
 }
 
-template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T, bool DIAG = false>
+// DIAG: 0 = a stage, 1 = stage 1 that also reduces the diagnostics of its
+// input, 2 = those diagnostics alone (no stage outputs written).
+template <int TX, int TY, int TZ, int PF, bool BETA, int S, typename T, int DIAG = 0>
 __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, int bid,
                                            unsigned char *lds_raw, int t, double *diag = nullptr, double sw = 0.0,
                                            double cw = 0.0)
@@ -1155,7 +1157,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 #pragma unroll
     for (int i = 0; i < 3; i++) o[17 + i] = ktY[i];
 
-    if constexpr (S < 4) {
+    if constexpr (S < 4 && DIAG == 2) {
+        // diagnostics alone: the stage outputs (and the arithmetic only they
+        // need) are dropped
+    } else if constexpr (S < 4) {
 #pragma unroll
         for (int m = 0; m < NK; m++) stn(b.kout + (long)m * b.n3, qb, o[m]);
         if constexpr (stores_next(PF, S)) {
@@ -1272,7 +1277,7 @@ stage_brick(StageBufs<T> b, BrickGeo g, Phys P, T dt)
 // per brick NDIAG partial sums to diag[brick * NDIAG + slot].  Two waves per
 // SIMD (the per-site slots in LDS leave room for two 256-thread or one
 // 512-thread workgroup per CU).
-template <int TX, int TY, int TZ, int PF, bool BETA, typename T = double>
+template <int TX, int TY, int TZ, int PF, bool BETA, typename T = double, int MODE = 1>
 __global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(waves_per_simd(false, PF, 1))))
 stage1_diag(StageBufs<T> b, BrickGeo g, Phys P, T dt, double *diag, double sw, double cw)
 {
@@ -1282,7 +1287,7 @@ stage1_diag(StageBufs<T> b, BrickGeo g, Phys P, T dt, double *diag, double sw, d
         const int per = (g.nbx * g.nby * g.nbz) >> 3;
         bid = (bid & 7) * per + (bid >> 3);
     }
-    brick_body<TX, TY, TZ, PF, BETA, 1, T, true>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag, sw, cw);
+    brick_body<TX, TY, TZ, PF, BETA, 1, T, MODE>(b, g, P, dt, bid, lds_raw, (int)threadIdx.x, diag, sw, cw);
 }
 
 template <int TX, int TY, int TZ>
@@ -1330,7 +1335,7 @@ hipError_t launch_tile_beta(int stage, const StageBufs<T> &b, const BrickGeo &g,
     return hipGetLastError();
 }
 
-template <int TX, int TY, int TZ, int PF, bool BETA, typename T>
+template <int TX, int TY, int TZ, int PF, bool BETA, typename T, int MODE = 1>
 hipError_t launch_diag_beta(const StageBufs<T> &b, const BrickGeo &g, const Phys &P, T dt, double *diag, double sw,
                             double cw, hipStream_t st)
 {
@@ -1341,14 +1346,15 @@ hipError_t launch_diag_beta(const StageBufs<T> &b, const BrickGeo &g, const Phys
     {
         std::lock_guard<std::mutex> gd(attr_mu);
         if (!attr_set) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&stage1_diag<TX, TY, TZ, PF, BETA, T>),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            const hipError_t e =
+                hipFuncSetAttribute(reinterpret_cast<const void *>(&stage1_diag<TX, TY, TZ, PF, BETA, T, MODE>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             attr_set = true;
         }
     }
-    stage1_diag<TX, TY, TZ, PF, BETA, T><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag, sw,
-                                                                                                cw);
+    stage1_diag<TX, TY, TZ, PF, BETA, T, MODE><<<dim3(g.nbx * g.nby * g.nbz), dim3(B::NT), lds, st>>>(b, g, P, dt, diag,
+                                                                                                      sw, cw);
     return hipGetLastError();
 }
 
@@ -1375,6 +1381,16 @@ hipError_t launch_tile_diag(const StageBufs<T> &b, const ewpt_lattice &L, const 
     const BrickGeo g = make_brick_geo(TX, TY, TZ, L, xcd_map, order, b.zghost);
     if (P.bw != 0.0 || P.by != 0.0) return launch_diag_beta<TX, TY, TZ, PF, true, T>(b, g, P, T(dt), diag, sw, cw, st);
     return launch_diag_beta<TX, TY, TZ, PF, false, T>(b, g, P, T(dt), diag, sw, cw, st);
+}
+
+// the diagnostics of b.y alone (stage 1's passes without its outputs; the
+// beta terms enter only the outputs)
+template <int TX, int TY, int TZ, int PF>
+hipError_t launch_tile_diag_only(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, int xcd_map,
+                                 int order, double *diag, double sw, double cw, hipStream_t st)
+{
+    const BrickGeo g = make_brick_geo(TX, TY, TZ, L, xcd_map, order, b.zghost);
+    return launch_diag_beta<TX, TY, TZ, PF, false, double, 2>(b, g, P, 0.0, diag, sw, cw, st);
 }
 
 template <int TX, int TY, int TZ, int PF, typename T = double>

@@ -39,6 +39,8 @@ EWPT_DECLARE_BRICK(34)
 #undef EWPT_DECLARE_BRICK
 hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                                  int order, double *diag, double sw, double cw, hipStream_t st);
+hipError_t launch_brick_v32_diag_only(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, int order,
+                                      double *diag, double sw, double cw, hipStream_t st);
 
 namespace {
 
@@ -97,6 +99,18 @@ hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewp
 {
     if (!brick_has_diag(variant)) return hipErrorInvalidValue;
     return launch_brick_v32_diag(b, L, P, dt, variant / 100, diag, sw, cw, st);
+}
+
+bool state_diag_fits(const ewpt_lattice &L) { return brick_supported(32, L); }
+
+hipError_t launch_state_diag(const double *y, long n3, const ewpt_lattice &L, const Phys &P, double *diag, double sw,
+                             double cw, hipStream_t st)
+{
+    if (!state_diag_fits(L)) return hipErrorInvalidValue;
+    StageBufs<double> b{};
+    b.y = y;
+    b.n3 = n3;
+    return launch_brick_v32_diag_only(b, L, P, 3, diag, sw, cw, st);   // brick cube order 3, as the stage kernels
 }
 
 hipError_t launch_stage_brick(int variant, int stage, const StageBufs<double> &b, const ewpt_lattice &L,

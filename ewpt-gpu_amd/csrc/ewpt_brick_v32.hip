@@ -23,4 +23,10 @@ hipError_t launch_brick_v32_diag(const StageBufs<double> &b, const ewpt_lattice 
     return launch_tile_diag<16, 4, 4, 211704432>(b, L, P, dt, 1, order, diag, sw, cw, st);
 }
 
+hipError_t launch_brick_v32_diag_only(const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, int order,
+                                      double *diag, double sw, double cw, hipStream_t st)
+{
+    return launch_tile_diag_only<16, 4, 4, 211704432>(b, L, P, 1, order, diag, sw, cw, st);
+}
+
 }  // namespace ewpt

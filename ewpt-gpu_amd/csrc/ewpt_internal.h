@@ -68,7 +68,11 @@ bool brick_has_diag(int variant);
 long diag_rows(const ewpt_lattice &L);   // rows of diag (bricks of the diagnostic stage 1)
 hipError_t launch_stage1_diag(int variant, const StageBufs<double> &b, const ewpt_lattice &L, const Phys &P, double dt,
                               double *diag, double sw, double cw, hipStream_t st);
-// sum (slot 8: min) nblk rows of a row-major [nblk][nslot] array, slot by slot in a fixed order
+// The same rows for an FP64 state y (36 * n3, periodic) without a stage: the
+// passes of variant 32's stage 1 with its outputs dropped, diag_rows(L) rows.
+bool state_diag_fits(const ewpt_lattice &L);
+hipError_t launch_state_diag(const double *y, long n3, const ewpt_lattice &L, const Phys &P, double *diag, double sw,
+                             double cw, hipStream_t st);
 // Fixed-order sum / min (slots in minmask) of nblk rows of partials, stride
 // doubles per row, slots 0..ns-1 (ns <= 32) into res[0..ns-1]; overwrites
 // some rows of part.
