@@ -1672,6 +1672,17 @@ int gauss_and_enorm(ewpt_ctx *c, const ewpt_params *p, double *G, double *En)
 {
     int rc = set_device(c);
     if (rc) return rc;
+    double rd[DIAG_SLOTS];
+    bool done = false;
+    if ((rc = state_diag_query(c, p, 0.0, rd, &done))) return rc;
+    if (done) {
+        double e[13], gs[8];
+        diag_outputs(c, rd, false, e, gs);
+        *G = std::sqrt(gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2] + gs[3] * gs[3]);
+        *En = std::sqrt(2.0 * (e[3] + e[5]));
+        if (!std::isfinite(*G) || !std::isfinite(*En)) return EWPT_ERR_NONFINITE;
+        return EWPT_OK;
+    }
     if ((rc = ensure_view(c))) return rc;
     HIPCHK(c, launch_gauss(c->y, c->lat, phys_of(*p), c->part, c->res, c->stream), "gauss launch");
     double r[10];
