@@ -204,7 +204,7 @@ auto with_kern(int v, F &&f)
     if constexpr (std::is_same_v<T, double>) {
         if (v == 2) return f(Kern<8, 8, 8, 1, T>{});
         if (v == 32) return f(Kern<16, 4, 4, 211704432, T>{});
-        if (v == 34) return f(Kern<16, 4, 4, 311704432, T>{});
+        if (v == 34) return f(Kern<16, 4, 4, 311704433, T>{});
     }
     return f(Kern<16, 8, 4, 4432, T>{});
 }

@@ -17,7 +17,7 @@
 //       arithmetic in the same order as 24 (bitwise equal results)
 //   34: 32 with y reconstructed from the stored stage inputs and derivatives
 //       (stage 4 reads no y, stage 2 reads k1 instead of y): fewer bytes, one
-//       more rounding per reconstructed value
+//       more rounding per reconstructed value; stage-4 prefetch depth 3
 // All use the XCD-aware block order; + 100 * o selects brick enumeration
 // order o (brick_coords).
 #include "ewpt_internal.h"
