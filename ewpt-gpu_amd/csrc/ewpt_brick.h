@@ -228,9 +228,12 @@ __device__ __forceinline__ void stn(T *base, unsigned boff, T v)
 {
     __builtin_nontemporal_store(v, (global_ptr<T>)(reinterpret_cast<char *>(base) + boff));
 }
-// Streaming load of a value no other brick reads: stage 4's update inputs
+// Streaming load of a value no other brick reads, so that it does not push
+// the halo lines neighbours re-read out of L2: stage 4's update inputs
 // (512^3: FP64 stage 4 33.4 -> 32.7 ms together with a prefetch depth of 3,
-// mixed 25.8 -> 25.6 ms).
+// mixed 25.8 -> 25.6 ms) and the y / k1 values of the next stage input that
+// stages 2-3 load during their last passes (FP64 stages 2 + 3 -0.5 ms, mixed
+// -0.7 ms).
 template <typename T>
 __device__ __forceinline__ T ldn(const T *base, unsigned boff)
 {
@@ -975,23 +978,23 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
             if constexpr (YK && pp >= NPASS - YP) {
                 constexpr int q = pp - (NPASS - YP);
 #pragma unroll
-                for (int m = q * NK / YP; m < (q + 1) * NK / YP; m++) kpre[m] = ldo(K(b.k1, m), qb);
+                for (int m = q * NK / YP; m < (q + 1) * NK / YP; m++) kpre[m] = ldn(K(b.k1, m), qb);
             } else if constexpr (YE && !YF && pp >= NPASS - YP) {
                 constexpr int q = pp - (NPASS - YP);
 #pragma unroll
-                for (int ch = q * NCH / YP; ch < (q + 1) * NCH / YP; ch++) ypre[ch] = ldo(Y(ch), qb);
+                for (int ch = q * NCH / YP; ch < (q + 1) * NCH / YP; ch++) ypre[ch] = ldn(Y(ch), qb);
             }
             if constexpr (YF && pp >= NPASS - YP) {
                 constexpr int q = pp - (NPASS - YP);
                 if constexpr (q < YP / 2) {
 #pragma unroll
                     for (int ch = q * 16 / (YP / 2); ch < (q + 1) * 16 / (YP / 2); ch++)
-                        stn(b.unext + (long)ch * b.n3, qb, R(ldo(Y(ch), qb)) + c[ch + 20] * (S == 3 ? dt : half_dt));
+                        stn(b.unext + (long)ch * b.n3, qb, R(ldn(Y(ch), qb)) + c[ch + 20] * (S == 3 ? dt : half_dt));
                 } else {
                     constexpr int q2 = q - YP / 2;
 #pragma unroll
                     for (int ch = 16 + q2 * 20 / (YP / 2); ch < 16 + (q2 + 1) * 20 / (YP / 2); ch++)
-                        ypre[ch] = ldo(Y(ch), qb);
+                        ypre[ch] = ldn(Y(ch), qb);
                 }
             }
             if constexpr (pp + PFS < NPASS) fetch(std::integral_constant<int, pp + PFS>{}, rr[pp % PFS]);
