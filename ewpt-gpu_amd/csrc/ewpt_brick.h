@@ -1195,7 +1195,7 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
 #pragma unroll
         for (int m = 0; m < NK; m++) {
             R nh, nl;
-            const R hi = lean_stash(PF) ? R(ldo(Y(16 + m), qb)) : nstash[(NK + m) * B::NT + t];
+            const R hi = lean_stash(PF) ? R(ldn(Y(16 + m), qb)) : nstash[(NK + m) * B::NT + t];
             two_sum(hi, o[m] * d6 + nstash[m * B::NT + t], nh, nl);
             stn(b.ynew + (long)(16 + m) * b.n3, qb, nh);
             stn(b.ylo_new + (long)(16 + m) * b.n3, qb, nl);
