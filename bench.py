@@ -611,6 +611,7 @@ def main(argv=None) -> int:
         "roofline": head["roofline"],
         "step_roofline_frac": head["step_roofline_frac"],
         "stage_ms": head["stage_ms"],
+        "stages": head["stages"],
         "snapshot": head["snapshot"],
         "energy_check": head["energy_check"],
     }
@@ -619,7 +620,8 @@ def main(argv=None) -> int:
         out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"], "variant": m["variant"],
                         "speedup_vs_fp64": m["value"] / head["value"], "dtype": "f32 storage/stencils, f64 reductions",
                         "roofline": m["roofline"], "step_roofline_frac": m["step_roofline_frac"],
-                        "stage_ms": m["stage_ms"], "snapshot": m["snapshot"], "energy_check": m["energy_check"],
+                        "stage_ms": m["stage_ms"], "stages": m["stages"], "snapshot": m["snapshot"],
+                        "energy_check": m["energy_check"],
                         "energy_rel_diff_vs_fp64": abs(m["energy_check"] - head["energy_check"]) / abs(head["energy_check"])}
     if len(finals) == 2:
         ref, mix = finals["fp64"], finals["mixed"]
