@@ -967,7 +967,10 @@ __device__ __forceinline__ void brick_body(const StageBufs<T> &b, const BrickGeo
     // u3 = y + k2 dt/2 with y = u2 - k1 dt/2 (momenta, Gamma, Sigma) and, for a
     // field, y_f = u2_f - y_pi dt/2, y_pi = u2_pi - kt1 dt/2
     constexpr bool YK = YE && recon_y(PF) && S == 2;
-    constexpr int YP = 8;
+    // passes that load those values: the last 8, the k1 values of YK in the
+    // last 4 (held over 8 passes they spilled the FP64 stage 2: 20 B per lane;
+    // 512^3: stage 2 31.0 -> 30.6 ms)
+    constexpr int YP = YK ? 4 : 8;
     R ypre[YE && !YK ? NCH : 1];
     R kpre[YK ? NK : 1];
     if constexpr (!paired_of(PF, S) && !SPREAD) {
