@@ -84,19 +84,19 @@ STAGE_BYTES = {pr: stage_bytes_per_site(pr, 0) for pr in ("fp64", "mixed")}
 # was measured; a kernel changed since then needs a new counter run.
 TRAFFIC_SOURCE = {
     "r1": "profiles/stage_traffic_512.md (library at commit be83b44 / 1ea9c7a)",
-    "r5": "profiles/stage_traffic_512.md (library at commit 7c2fe36)",
+    "r6": "profiles/stage_traffic_512.md (library at commit c01baa4)",
 }
 MEASURED_TRAFFIC = {
-    ("fp64", 334, 1): (1113, "r5"), ("fp64", 334, 2): (1309, "r5"), ("fp64", 334, 3): (1410, "r5"),
-    ("fp64", 334, 4): (1461, "r5"),
+    ("fp64", 334, 1): (1113, "r6"), ("fp64", 334, 2): (1275, "r6"), ("fp64", 334, 3): (1410, "r6"),
+    ("fp64", 334, 4): (1466, "r6"),
     ("fp64", 332, 1): (1110, "r1"), ("fp64", 332, 2): (1504, "r1"), ("fp64", 332, 3): (1500, "r1"),
     ("fp64", 332, 4): (1882, "r1"),
     ("fp64", 324, 1): (1154, "r1"), ("fp64", 324, 2): (1387, "r1"), ("fp64", 324, 3): (1386, "r1"),
     ("fp64", 324, 4): (2407, "r1"),
     ("fp64", 311, 1): (849, "r1"), ("fp64", 311, 2): (1630, "r1"), ("fp64", 311, 3): (2212, "r1"),
     ("fp64", 311, 4): (3725, "r1"),
-    ("mixed", 333, 1): (598, "r5"), ("mixed", 333, 2): (674, "r5"), ("mixed", 333, 3): (672, "r5"),
-    ("mixed", 333, 4): (1214, "r5"),
+    ("mixed", 333, 1): (602, "r6"), ("mixed", 333, 2): (674, "r6"), ("mixed", 333, 3): (676, "r6"),
+    ("mixed", 333, 4): (1214, "r6"),
     ("mixed", 331, 1): (632, "r1"), ("mixed", 331, 2): (850, "r1"), ("mixed", 331, 3): (845, "r1"),
     ("mixed", 331, 4): (1495, "r1"),
     ("mixed", 315, 1): (628, "r1"), ("mixed", 315, 2): (696, "r1"), ("mixed", 315, 3): (698, "r1"),
