@@ -90,6 +90,27 @@ def test_energies_gauss(shape):
     np.testing.assert_allclose(g, gr, rtol=1e-10, atol=1e-15)
 
 
+@pytest.mark.parametrize("precision", [ewpt.EWPT_FP64, ewpt.EWPT_MIXED])
+def test_diagnostics_with_beta_terms(precision):
+    """The energies, Gauss norms and EM columns of a resident state come from
+    the diagnostic stage-1 pass without its stage outputs, instantiated
+    without the beta terms (they enter only the stage outputs): with beta_W,
+    beta_Y, gamma and J_ext on the results equal those with them off and the
+    oracle's."""
+    shape = (32, 16, 24)
+    s = fields.smooth_state(shape, seed=33)
+    p0 = fields.params()
+    p1 = fields.params(beta_w=0.2, beta_y=0.1, gamma=0.3, jex=0.5)
+    th = 0.49
+    with ewpt.Lattice(shape, DX, precision=precision) as lat:
+        lat.set_state(s)
+        e0, g0, m0 = lat.energies(p0), lat.gauss(p0), lat.em_energies(p0, th)
+        e1, g1, m1 = lat.energies(p1), lat.gauss(p1), lat.em_energies(p1, th)
+    assert np.array_equal(e0, e1) and np.array_equal(g0, g1) and np.array_equal(m0, m1)
+    np.testing.assert_allclose(e1, cref.energies(s, p1, DX), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(g1, cref.gauss(s, p1, DX), rtol=1e-10, atol=1e-15)
+
+
 def test_reductions_repeatable_partial_chunk():
     """1105 brick rows (80x52x68 on 16x4x4 bricks): the fixed-order reduction
     takes 553 chunks of 2 rows, the last one holding a single row.  The sums
